@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""meshFT forward + adjoint throughput on one MI355X.
+
+Workload (one "step"): the 20,480-face icosphere (subdivision 5, radial
+spherical-harmonic displacement l <= 6, c ~ N(0, 0.03), R = 1, seed 2000) in a
+200^3 box under a Gaussian OTF Sigma = 1e-3 I with band threshold 1e-5
+(~815k band frequencies): one meshFT forward and one adjoint through the C-ABI
+on inputs already resident in HBM (vertices, faces, band frequencies, an
+upstream gradient).  value = faces * band / step time, in G face.freq/s.
+
+Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs on
+one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "meshFT fwd+bwd Gface·freq/s; fit time to loss tol, learned vs plain Adam"
+UNIT = "Gface·freq/s"
+FP32_PEAK_TFLOPS = 157.3
+FLOPS_FWD = 20   # algorithmic FP32 flops per face.frequency pair, forward
+FLOPS_BWD = 85   # adjoint
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, xi_np, Vs, n_sample: int):
+    """Reference-semantics FP32 restatement (oracle), fwd + adjoint, all host cores."""
+    from oracle import oracle as O
+    from deltamic_amd import synthetic as syn
+    cores = min(16, os.cpu_count() or 1)
+    sel = np.sort(syn.rng(w.seed + 1).choice(len(xi_np), n_sample, replace=False))
+    xi = xi_np[sel]
+    g = syn.rng(w.seed + 4)
+    grad = (g.standard_normal(n_sample) + 1j * g.standard_normal(n_sample)).astype(np.complex64)
+    t0 = time.perf_counter()
+    ft = O.ref_forward(Vs, w.faces, w.coeff, xi, np.float32, threads=cores)
+    t1 = time.perf_counter()
+    O.ref_backward(Vs, w.faces, w.coeff, xi, grad, ft, np.float32, threads=cores)
+    t2 = time.perf_counter()
+    pairs = len(w.faces) * n_sample
+    return {
+        "value": pairs / (t2 - t0) / 1e9,
+        "unit": UNIT,
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"same mesh, {n_sample} of the band frequencies (seed {w.seed + 1}); "
+                   f"reference-semantics FP32 forward {t1 - t0:.2f} s + adjoint {t2 - t1:.2f} s "
+                   f"(oracle/meshft_oracle.cpp, g++ -O3 -fopenmp)"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-freqs", type=int, default=2048,
+                    help="band frequencies in the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--module-steps", type=int, default=3,
+                    help="extra end-to-end module steps (forward, render, backward) to time")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs meshFT on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}), flush=True)
+        return 2
+
+    import torch
+    from deltamic_amd import (Fourier3dMesh, generate_gaussian_psf, normalize_tensor,
+                              render_image_from_ftmesh, _lib)
+    from deltamic_amd import synthetic as syn
+    from deltamic_amd.hip_class import compute_band
+
+    dev = torch.device("cuda:0")
+    w = syn.config2()
+    t0 = time.perf_counter()
+    box = torch.tensor(w.box_size)
+    probe = Fourier3dMesh(box, w.box_shape, device=dev, dtype=torch.float32)
+    otf = generate_gaussian_psf(torch.tensor(w.sigma, device=dev), probe.xi0, probe.xi1, probe.xi2)
+    band = compute_band(otf, w.thresh, probe.xi0, probe.xi1, probe.xi2)
+    n = band.idx.numel()
+    nf, nv = len(w.faces), len(w.verts)
+    Vs = (torch.tensor(w.verts) - probe.box_size.cpu()[:, 0]).contiguous()
+    V = Vs.to(dev)
+    Fc = torch.tensor(w.faces, device=dev)
+    C = torch.tensor(w.coeff, device=dev)
+    g = syn.rng(w.seed + 4)
+    grad = torch.tensor((g.standard_normal(n) + 1j * g.standard_normal(n)).astype(np.complex64),
+                        device=dev)
+    out = torch.empty((n,), dtype=torch.complex64, device=dev)
+    gv = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+    L = _lib.lib()
+    wsf = _lib.workspace(L.dmic_meshft_forward_workspace(nv, nf, n), dev)
+    wsb = _lib.workspace(L.dmic_meshft_backward_workspace(nv, nf, n), dev)
+    st = torch.cuda.current_stream(dev).cuda_stream
+    log(f"setup {time.perf_counter() - t0:.1f} s: nv={nv} nf={nf} band={n}")
+
+    def step():
+        rc = L.dmic_meshft_forward(V.data_ptr(), nv, Fc.data_ptr(), 1, nf, C.data_ptr(),
+                                   band.xi_list.data_ptr(), n, out.data_ptr(), wsf.data_ptr(),
+                                   wsf.numel(), st)
+        _lib.check(rc, "forward")
+        rc = L.dmic_meshft_backward(V.data_ptr(), nv, Fc.data_ptr(), 1, nf, C.data_ptr(),
+                                    grad.data_ptr(), out.data_ptr(), band.xi_list.data_ptr(), n,
+                                    gv.data_ptr(), wsb.data_ptr(), wsb.numel(), st)
+        _lib.check(rc, "backward")
+
+    for i in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"warmup {i + 1}/{args.warmup}")
+    L.dmic_profile_enable(1)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    t_wall = time.perf_counter()
+    for i in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    ms_total = e0.elapsed_time(e1)
+    fwd_ms, fwd_n, bwd_ms, bwd_n = (ctypes.c_double(), ctypes.c_long(), ctypes.c_double(),
+                                    ctypes.c_long())
+    L.dmic_profile_read(ctypes.byref(fwd_ms), ctypes.byref(fwd_n), ctypes.byref(bwd_ms),
+                        ctypes.byref(bwd_n))
+    L.dmic_profile_enable(0)
+    ms_step = ms_total / args.steps
+    pairs = nf * n
+    value = pairs / (ms_step * 1e-3) / 1e9
+    kf = fwd_ms.value / max(fwd_n.value, 1)
+    kb = bwd_ms.value / max(bwd_n.value, 1)
+    log(f"step {ms_step:.3f} ms (wall {t_wall / args.steps * 1e3:.3f} ms): fwd kernel {kf:.3f} ms, "
+        f"bwd kernel {kb:.3f} ms")
+    ach_f = FLOPS_FWD * pairs / (kf * 1e-3) / 1e12
+    ach_b = FLOPS_BWD * pairs / (kb * 1e-3) / 1e12
+    dom = ("meshft_bwd_kernel", ach_b, kb) if kb >= kf else ("meshft_fwd_kernel", ach_f, kf)
+
+    # end-to-end module step: forward, render (hipFFT), normalize, loss, backward
+    module_ms = None
+    if args.module_steps > 0:
+        mesh_ft = Fourier3dMesh(box, w.box_shape, dev, torch.float32, otf, w.thresh)
+        Vp = torch.tensor(w.verts, device=dev, requires_grad=True)
+
+        def mstep():
+            Vp.grad = None
+            img = normalize_tensor(render_image_from_ftmesh(mesh_ft(Vp, Fc, C), otf, w.box_shape))
+            img.sum().backward()
+
+        mstep()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.module_steps):
+            mstep()
+        b.record()
+        torch.cuda.synchronize()
+        module_ms = a.elapsed_time(b) / args.module_steps
+        log(f"module step (fwd+render+normalize+bwd): {module_ms:.3f} ms")
+
+    cpu = None
+    if args.cpu_freqs > 0:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(w, band.xi_list.cpu().numpy(), Vs.numpy(), args.cpu_freqs)
+        log(f"cpu baseline {cpu['value']:.4f} {UNIT} on {cpu['cores']} threads")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("icosphere s5 (10242 v, 20480 f) + real SH l<=6 c~N(0,0.03), R=1, seed 2000; "
+                         "200^3 box (compute_box_size offset 0.2); Gaussian OTF Sigma=1e-3 I; "
+                         "narrowband_thresh 1e-5; one meshFT forward + adjoint via the C-ABI"),
+            "faces": nf,
+            "verts": nv,
+            "band": n,
+            "box": list(w.box_shape),
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": dom[0],
+            "achieved": round(dom[1], 3),
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(dom[1] / FP32_PEAK_TFLOPS, 4),
+            "traffic": None,
+            "flops_per_pair": FLOPS_BWD if dom[0] == "meshft_bwd_kernel" else FLOPS_FWD,
+            "kernel_ms": round(dom[2], 4),
+        },
+        "kernels": {
+            "meshft_fwd_kernel_ms": round(kf, 4),
+            "meshft_bwd_kernel_ms": round(kb, 4),
+            "fwd_tflops": round(ach_f, 3),
+            "bwd_tflops": round(ach_b, 3),
+            "module_step_ms": None if module_ms is None else round(module_ms, 3),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

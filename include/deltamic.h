@@ -1,0 +1,144 @@
+/*
+ * deltamic_amd C-ABI: the narrow-band analytic Fourier transform of triangle
+ * meshes ("meshFT") and its adjoint, as hand-written HIP kernels for gfx950.
+ *
+ * Every entry point takes plain device pointers and sizes, enqueues its
+ * kernels on `stream` (no device-wide synchronisation, no allocation: scratch
+ * memory is a caller-provided `workspace` whose size the matching *_workspace
+ * function returns) and returns DMIC_OK or a negative error code whose text
+ * dmic_last_error() returns.  Results are deterministic: no floating-point
+ * atomics; every reduction has a fixed order.
+ *
+ * Layouts (all row-major, device memory):
+ *   verts       float [nv][3]   vertex positions already shifted to the box
+ *                                minimum (render.py:148 does Verts - box_size[:,0])
+ *   faces       int64 or int32 [nf][3] (faces_is_int64 selects)
+ *   faces_coeff float [nf]
+ *   xi_list     float [n][3]    compacted band frequencies
+ *   ftmesh_list float [n][2]    complex64 band values (re, im)
+ *
+ * Face indices outside [0, nv) do not fault: the vertex reads as the origin
+ * and DMIC_ERR_INDEX is reported by dmic_check_index_flag().
+ */
+#ifndef DELTAMIC_AMD_H
+#define DELTAMIC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMIC_OK 0
+#define DMIC_ERR_ARG (-1)
+#define DMIC_ERR_WORKSPACE (-2)
+#define DMIC_ERR_HIP (-3)
+#define DMIC_ERR_INDEX (-4)
+
+/* Library version (major*10000 + minor*100 + patch). */
+int dmic_version(void);
+
+/* Text of the last error reported on the calling thread ("" if none). */
+const char* dmic_last_error(void);
+
+/* ------------------------------------------------------------------ meshFT */
+
+/* Workspace bytes dmic_meshft_forward needs for these sizes. */
+size_t dmic_meshft_forward_workspace(int64_t nv, int64_t nf, int64_t n);
+
+/*
+ * ftmesh_list[m] = sum_t w_t F(xi_m . v_a, xi_m . v_b, xi_m . v_c),
+ *   w_t = 2 c_t A_t / sum_s c_s A_s,  F = 2nd divided difference of -exp(-ix).
+ * Replaces forward_fourier3d_cpp -> fourier3d_forward -> fourier3d_cuda_forward
+ * (deltamic/cpp/fourier3d_cuda.cpp:27-42, fourier3d_cuda_kernel.cu:421-462).
+ * n == 0 or nf == 0 is valid (zero output / nothing to do).
+ */
+int dmic_meshft_forward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                        int64_t nf, const float* faces_coeff, const float* xi_list, int64_t n,
+                        float* ftmesh_list, void* workspace, size_t workspace_bytes,
+                        hipStream_t stream);
+
+/* Workspace bytes dmic_meshft_backward needs for these sizes. */
+size_t dmic_meshft_backward_workspace(int64_t nv, int64_t nf, int64_t n);
+
+/*
+ * grad_verts[v] = d/dv sum_m Re(conj(g_m) ftmesh_m), g = grad_output_list
+ * (torch's complex gradient convention, fourier3d_cuda_kernel.cu:392-394).
+ * Per-face partial sums over frequency tiles, then a fixed-order CSR
+ * face->vertex reduction (any vertex valence).  ftmesh_list is accepted for
+ * signature compatibility; the area-normalisation term is rebuilt from the
+ * per-face sums.  Replaces backward_fourier3d_cpp -> fourier3d_backward ->
+ * fourier3d_cuda_backward (fourier3d_cuda.cpp:44-65, fourier3d_cuda_kernel.cu:466-539).
+ */
+int dmic_meshft_backward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                         int64_t nf, const float* faces_coeff, const float* grad_output_list,
+                         const float* ftmesh_list, const float* xi_list, int64_t n,
+                         float* grad_verts, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream);
+
+/*
+ * Reads (synchronously, on `stream`) the out-of-range face index flag that
+ * the last dmic_meshft_forward / dmic_meshft_backward call on `workspace`
+ * set (each call clears it first).  Returns DMIC_OK or DMIC_ERR_INDEX.  No
+ * counterpart in the reference, which does not check indices.
+ */
+int dmic_check_index_flag(void* workspace, hipStream_t stream);
+
+/*
+ * Per-kernel timing for benchmarks (no counterpart in the reference): when
+ * enabled, hipEvents are recorded on the launch stream around the main
+ * forward kernel and the main adjoint kernel of every call.  Enabling also
+ * resets the totals.  dmic_profile_read waits for the last recorded launch
+ * and returns the summed milliseconds and launch counts.  Not thread-safe.
+ */
+int dmic_profile_enable(int enable);
+int dmic_profile_read(double* fwd_ms, long* fwd_count, double* bwd_ms, long* bwd_count);
+
+/* -------------------------------------------------------- narrow band */
+
+/* Workspace bytes for dmic_band_count / dmic_band_fill on an n0 x n1 x n2 box. */
+size_t dmic_band_workspace(int64_t n0, int64_t n1, int64_t n2);
+
+/*
+ * Counts the band voxels filter[i] > thresh (no abs, as cuda_class.py:21) and
+ * writes the count to *count_dev (device int64).  Together with
+ * dmic_band_fill this replaces the boolean-mask gather of cuda_class.py:20-23.
+ */
+int dmic_band_count(const float* filter, int64_t n0, int64_t n1, int64_t n2, float thresh,
+                    int64_t* count_dev, void* workspace, size_t workspace_bytes,
+                    hipStream_t stream);
+
+/*
+ * After dmic_band_count on the same workspace: writes the band in row-major
+ * voxel order: idx[j] = linear voxel index, xi_list[j] = (xi0[i0], xi1[i1],
+ * xi2[i2]) (the meshgrid 'ij' gather of cuda_class.py:20-23).
+ */
+int dmic_band_fill(const float* filter, int64_t n0, int64_t n1, int64_t n2, float thresh,
+                   const float* xi0, const float* xi1, const float* xi2, int32_t* idx,
+                   float* xi_list, void* workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* box[:] = 0 then box[idx[j]] = vals[j]: the scatter ftmesh[mask] = ftmesh_list
+ * of cuda_class.py:27-28.  vals and box are complex64 as float pairs. */
+int dmic_band_scatter(const float* vals, const int32_t* idx, int64_t n, float* box, int64_t nbox,
+                      hipStream_t stream);
+
+/* vals[j] = box[idx[j]]: the gather grad_output[mask] of cuda_class.py:36. */
+int dmic_band_gather(const float* box, const int32_t* idx, int64_t n, float* vals,
+                     hipStream_t stream);
+
+/*
+ * out[i0][i1][i2] = exp(-xi^T Sigma xi / 2), xi = (xi0[i0], xi1[i1], xi2[i2]);
+ * sigma is a HOST array of 9 floats (row-major 3x3).  Replaces
+ * generate_gaussian_psf (deltamic/psf.py:15-20).
+ */
+int dmic_gaussian_otf(const float* sigma_host, const float* xi0, const float* xi1,
+                      const float* xi2, int64_t n0, int64_t n1, int64_t n2, float* out,
+                      hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DELTAMIC_AMD_H */

@@ -1,0 +1,165 @@
+"""GPU parity of the HIP meshFT path against the CPU oracle (oracle/).
+
+Tolerances (rel-L2 over the band or over all vertices):
+  forward  <= 1e-5 vs the accurate FP64 oracle,
+  adjoint  <= 1e-4 vs the accurate FP64 oracle,
+  band / scatter / gather / face-index bookkeeping: exact.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.conftest import threads
+from deltamic_amd import synthetic as syn
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-5
+BWD_TOL = 1e-4
+
+
+def _mesh(w, device):
+    V = torch.tensor(w.verts, device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.tensor(w.coeff, device=device)
+    return V, Fc, C
+
+
+def _band(w, device):
+    """GPU band of the workload's Gaussian OTF, checked against numpy on the same OTF."""
+    from deltamic_amd import generate_gaussian_psf, spatial_frequency_axes
+    from deltamic_amd.hip_class import compute_band
+    bs = torch.tensor(w.box_size, dtype=torch.float32, device=device)
+    axes = spatial_frequency_axes(bs, w.box_shape, torch.float32, device)
+    otf = generate_gaussian_psf(torch.tensor(w.sigma, device=device), *axes)
+    band = compute_band(otf, w.thresh, *axes)
+    return bs, axes, otf, band
+
+
+def _shifted(w, bs):
+    # Verts - box_size[:, 0] in FP32, as Fourier3dMesh.forward does (render.py:148)
+    return (torch.tensor(w.verts) - bs.cpu()[:, 0]).numpy()
+
+
+def test_band_and_otf_match_numpy(device):
+    w = syn.config1()
+    bs, axes, otf, band = _band(w, device)
+    ax_np = O.frequency_axes(bs.cpu().numpy(), w.box_shape)
+    for a, b in zip(axes, ax_np):
+        assert np.array_equal(a.cpu().numpy(), b)
+    ref = O.gaussian_otf(w.sigma.astype(np.float64), *ax_np)
+    got = otf.cpu().numpy().astype(np.float64)
+    # FP32 exponent q has relative error ~1e-7, so exp(-q/2) has ~1e-7 (1 + q) relative error
+    live = ref > 1e-30
+    q = -2.0 * np.log(ref[live])
+    relerr = np.abs(got[live] - ref[live]) / ref[live]
+    assert np.all(relerr <= 2e-7 * (4.0 + q)), relerr.max()
+    assert np.all(got[~live] < 1e-29)
+    idx, xi = O.band(otf.cpu().numpy(), np.float32(w.thresh), *ax_np)
+    assert np.array_equal(band.idx.cpu().numpy().astype(np.int64), idx)
+    assert np.array_equal(band.xi_list.cpu().numpy(), xi)
+    assert 20000 < len(idx) < 24000
+
+
+def test_band_random_filter_exact(device):
+    from deltamic_amd.hip_class import compute_band
+    g = np.random.default_rng(7)
+    for shape in [(1, 1, 1), (3, 5, 7), (33, 17, 65), (64, 64, 64)]:
+        f = g.random(shape, dtype=np.float32)
+        axes = [torch.tensor(g.standard_normal(n).astype(np.float32), device=device) for n in shape]
+        band = compute_band(torch.tensor(f, device=device), 0.5, *axes)
+        idx, xi = O.band(f, np.float32(0.5), *[a.cpu().numpy() for a in axes])
+        assert np.array_equal(band.idx.cpu().numpy().astype(np.int64), idx)
+        assert np.array_equal(band.xi_list.cpu().numpy(), xi)
+
+
+def test_forward_cfg1_full_band(device):
+    from deltamic_amd.fourier3d_cpp import forward_fourier3d_cpp
+    w = syn.config1()
+    bs, axes, otf, band = _band(w, device)
+    Vs = _shifted(w, bs)
+    out = forward_fourier3d_cpp(torch.tensor(Vs, device=device), torch.tensor(w.faces, device=device),
+                                torch.tensor(w.coeff, device=device), band.xi_list)
+    xi = band.xi_list.cpu().numpy()
+    ref = O.acc_forward(Vs, w.faces, w.coeff, xi, threads=threads())
+    err = O.rel_l2(out.cpu().numpy(), ref)
+    print(f"cfg1 forward rel-L2 vs FP64 oracle: {err:.3e} (n={len(xi)})")
+    assert err <= FWD_TOL
+
+
+def test_forward_cfg2_subset(device):
+    """20480 faces, 200^3 box: 4096 band frequencies drawn by seed + 1."""
+    from deltamic_amd.fourier3d_cpp import forward_fourier3d_cpp
+    w = syn.config2()
+    bs, axes, otf, band = _band(w, device)
+    n = band.idx.numel()
+    assert 780000 < n < 850000
+    sel = np.sort(syn.rng(w.seed + 1).choice(n, 4096, replace=False))
+    xi_sub = band.xi_list[torch.tensor(sel, device=device)].contiguous()
+    Vs = _shifted(w, bs)
+    out = forward_fourier3d_cpp(torch.tensor(Vs, device=device), torch.tensor(w.faces, device=device),
+                                torch.tensor(w.coeff, device=device), xi_sub)
+    ref = O.acc_forward(Vs, w.faces, w.coeff, xi_sub.cpu().numpy(), threads=threads())
+    err = O.rel_l2(out.cpu().numpy(), ref)
+    print(f"cfg2 forward rel-L2 vs FP64 oracle (4096 freqs): {err:.3e}")
+    assert err <= FWD_TOL
+
+
+def _grad(n, seed):
+    g = syn.rng(seed)
+    return (g.standard_normal(n) + 1j * g.standard_normal(n)).astype(np.complex64)
+
+
+def test_backward_cfg1_full_band(device):
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    w = syn.config1()
+    bs, axes, otf, band = _band(w, device)
+    Vs = _shifted(w, bs)
+    V = torch.tensor(Vs, device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.tensor(w.coeff, device=device)
+    n = band.idx.numel()
+    g = _grad(n, w.seed + 4)
+    ft = forward_fourier3d_cpp(V, Fc, C, band.xi_list)
+    gv = backward_fourier3d_cpp(V, Fc, C, torch.tensor(g, device=device), ft, band.xi_list)
+    ref = O.acc_backward(Vs, w.faces, w.coeff, band.xi_list.cpu().numpy(), g, threads=threads())
+    err = O.rel_l2(gv.cpu().numpy(), ref)
+    print(f"cfg1 adjoint rel-L2 vs FP64 oracle: {err:.3e}")
+    assert err <= BWD_TOL
+
+
+def test_backward_cfg2_subset(device):
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    w = syn.config2()
+    bs, axes, otf, band = _band(w, device)
+    n = band.idx.numel()
+    sel = np.sort(syn.rng(w.seed + 1).choice(n, 4096, replace=False))
+    xi_sub = band.xi_list[torch.tensor(sel, device=device)].contiguous()
+    Vs = _shifted(w, bs)
+    V = torch.tensor(Vs, device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.tensor(w.coeff, device=device)
+    g = _grad(4096, w.seed + 4)
+    ft = forward_fourier3d_cpp(V, Fc, C, xi_sub)
+    gv = backward_fourier3d_cpp(V, Fc, C, torch.tensor(g, device=device), ft, xi_sub)
+    ref = O.acc_backward(Vs, w.faces, w.coeff, xi_sub.cpu().numpy(), g, threads=threads())
+    err = O.rel_l2(gv.cpu().numpy(), ref)
+    print(f"cfg2 adjoint rel-L2 vs FP64 oracle (4096 freqs): {err:.3e}")
+    assert err <= BWD_TOL
+
+
+def test_bitwise_deterministic(device):
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    w = syn.config1()
+    bs, axes, otf, band = _band(w, device)
+    V = torch.tensor(_shifted(w, bs), device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.tensor(w.coeff, device=device)
+    g = torch.tensor(_grad(band.idx.numel(), 5), device=device)
+    a = forward_fourier3d_cpp(V, Fc, C, band.xi_list)
+    b = forward_fourier3d_cpp(V, Fc, C, band.xi_list)
+    assert torch.equal(a.view(torch.float32), b.view(torch.float32))
+    ga = backward_fourier3d_cpp(V, Fc, C, g, a, band.xi_list)
+    gb = backward_fourier3d_cpp(V, Fc, C, g, a, band.xi_list)
+    assert torch.equal(ga, gb)
