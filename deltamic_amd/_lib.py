@@ -39,6 +39,14 @@ SIGNATURES = {
     "dmic_band_scatter": ([_vp, _vp, _i64, _vp, _i64, _vp], _int),
     "dmic_band_gather": ([_vp, _vp, _i64, _vp, _vp], _int),
     "dmic_gaussian_otf": ([_vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp], _int),
+    "dmic_gemm_bf16": ([_vp, _int, _i64, _i64, _vp, _int, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
+                        _vp, _vp, _i64, _int, _vp], _int),
+    "dmic_mlp_weight_count": ([], _i64),
+    "dmic_mlp_bias_count": ([], _i64),
+    "dmic_mlp_forward": ([_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp], _int),
+    "dmic_mlp_backward": ([_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp], _int),
+    "dmic_adam_step": ([_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i64, _vp], _int),
+    "dmic_to_bf16": ([_vp, _i64, _vp, _vp], _int),
 }
 
 _LIB = None

@@ -1,0 +1,146 @@
+"""Learned-proposal fit vs plain Adam (config 5), end to end on one GPU.
+
+    python -m deltamic_amd.learned_fit --train-fits 24 --train-steps 300 \
+        --eval-shapes 4 --eval-steps 1000 --out results/learned
+
+1. Trajectories: ``--train-fits`` seeded config-4 fits (target seeds
+   4000 + k) run plain Adam for ``--train-steps`` steps; every ``--every``
+   steps the features and the displacement over the next ``--horizon`` steps
+   are recorded.
+2. Training: the proposal MLP (HIP kernels) is trained on those pairs.
+3. Evaluation on held-out target seeds (4000 + 10000 + k): plain Adam for
+   ``--eval-steps`` steps sets the reference final loss and wall time; the
+   accept/reject learned fit runs until it reaches that loss.  The ratio of
+   wall times is the reported speed-up.
+
+Writes ``summary.json`` (per shape: losses, times, acceptance) and the
+network weights (``proposal.pt``) under ``--out``.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import synthetic as syn
+from .fit import fit_adam, make_problem
+from .proposal import Features, ProposalMLP, fit_learned, trajectory_samples
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def collect(args, device):
+    xs, ts = [], []
+    for k in range(args.train_fits):
+        t0 = time.perf_counter()
+        w = syn.config4(k, subdivisions=args.subdiv, box=args.box)
+        prob = make_problem(w, device)
+        feats = Features(w.faces, len(w.extra["init_verts"]), prob.lap, device)
+        d = trajectory_samples(prob, feats, args.train_steps, args.every, args.horizon)
+        xs.append(d.x)
+        ts.append(d.target)
+        log(f"trajectory {k + 1}/{args.train_fits}: {d.x.shape[0]} rows "
+            f"({time.perf_counter() - t0:.1f} s)")
+    return torch.cat(xs), torch.cat(ts)
+
+
+def train(net, x, t, epochs, batch, lr, seed):
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    n = x.shape[0]
+    hist = []
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g).to(x.device)
+        tot = 0.0
+        nb = 0
+        for i in range(0, n - batch + 1, batch):
+            idx = perm[i:i + batch]
+            tot += float(net.train_step(x[idx].contiguous(), t[idx].contiguous(), lr=lr))
+            nb += 1
+        hist.append(tot / max(nb, 1))
+        log(f"epoch {ep + 1}/{epochs}: loss {hist[-1]:.5f}")
+    return hist
+
+
+def evaluate(net, args, device, subdiv, k):
+    w = syn.config4(10000 + k, subdivisions=subdiv, box=args.box)
+    prob = make_problem(w, device)
+    feats = Features(w.faces, len(w.extra["init_verts"]), prob.lap, device)
+    # warm the band cache and kernels outside the timed fits
+    prob.loss_and_grad(prob.init_verts)
+    torch.cuda.synchronize()
+    adam = fit_adam(prob, steps=args.eval_steps, sync_every=1)
+    target = adam.losses[-1]
+    t_adam = adam.times[-1]
+    learned = fit_learned(prob, net, feats, steps=args.eval_steps, stop_loss=target)
+    reached = bool(learned.losses and learned.losses[-1] <= target)
+    t_learned = learned.times[-1] if learned.times else float("nan")
+    # wall time plain Adam needed to first reach the same loss is t_adam (its final loss)
+    rec = {
+        "faces": int(len(w.faces)), "seed": w.seed, "adam_final_loss": target,
+        "adam_wall_s": t_adam, "adam_steps": args.eval_steps,
+        "learned_reached": reached, "learned_wall_s": t_learned,
+        "learned_iters": len(learned.losses), "learned_final_loss": learned.losses[-1],
+        "accepted": learned.accepted, "proposals": learned.proposals,
+        "speedup": (t_adam / t_learned) if reached else None,
+        "adam_loss_curve": adam.losses[::50], "learned_loss_curve": learned.losses[::10],
+    }
+    log(f"eval faces={rec['faces']} seed={w.seed}: adam {t_adam:.2f} s -> {target:.6f}; learned "
+        f"{t_learned:.2f} s ({rec['learned_iters']} it, {learned.accepted} accepted) "
+        f"speedup {rec['speedup']}")
+    return rec
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train-fits", type=int, default=24)
+    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--every", type=int, default=10)
+    ap.add_argument("--horizon", type=int, default=100)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--eval-shapes", type=int, default=4)
+    ap.add_argument("--eval-shapes-large", type=int, default=1)
+    ap.add_argument("--eval-steps", type=int, default=1000)
+    ap.add_argument("--subdiv", type=int, default=4)
+    ap.add_argument("--box", type=int, default=128)
+    ap.add_argument("--out", default="gpurun_out/learned")
+    ap.add_argument("--load", default=None, help="skip training; load proposal.pt")
+    args = ap.parse_args(argv)
+    os.makedirs(args.out, exist_ok=True)
+    device = torch.device("cuda:0")
+    net = ProposalMLP(device, seed=0)
+    summary = {"args": vars(args)}
+    if args.load:
+        net.load_state_dict(torch.load(args.load, weights_only=True))
+    else:
+        t0 = time.perf_counter()
+        x, t = collect(args, device)
+        summary["train_rows"] = int(x.shape[0])
+        summary["collect_s"] = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        summary["train_loss"] = train(net, x, t, args.epochs, args.batch, args.lr, seed=1)
+        summary["train_s"] = time.perf_counter() - t1
+        torch.save(net.state_dict(), os.path.join(args.out, "proposal.pt"))
+    evals = []
+    for k in range(args.eval_shapes):
+        evals.append(evaluate(net, args, device, args.subdiv, k))
+    for k in range(args.eval_shapes_large):
+        evals.append(evaluate(net, args, device, args.subdiv + 1, 100 + k))
+    summary["eval"] = evals
+    sp = [e["speedup"] for e in evals if e["speedup"] is not None]
+    summary["median_speedup"] = float(np.median(sp)) if sp else None
+    summary["reached"] = f"{len(sp)}/{len(evals)}"
+    with open(os.path.join(args.out, "summary.json"), "w") as f:
+        json.dump(summary, f, indent=1)
+    log(json.dumps({k: summary[k] for k in ("median_speedup", "reached")}))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,228 @@
+"""Learned per-vertex displacement proposals for the mesh fit.
+
+No counterpart in the reference.  A small MLP (12 -> 128 -> 128 -> 128 -> 3,
+bf16 weights, FP32 accumulation; HIP kernels in csrc/mlp.hip) maps
+per-vertex features
+
+    position (centred, scaled by the mesh RMS radius), unit vertex normal,
+    loss gradient and its 1-ring mean (both scaled by the mesh RMS gradient)
+
+to a displacement.  The network is trained on the method's own plain-Adam
+fitting trajectories: the target for a snapshot (V_k, grad_k) is the
+displacement Adam makes over the next ``horizon`` steps.  In the fit, a
+proposal V + d is accepted only if the meshFT + OTF loss decreases; otherwise
+a plain Adam step is taken (fit_learned).
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .fit import FitProblem, FitResult
+
+N_IN, N_HID, N_OUT = 12, 128, 3
+
+
+def vertex_face_table(faces: np.ndarray, nv: int):
+    """Incident faces per vertex, padded ([nv, K] int64, mask [nv, K] float32)."""
+    inc = [[] for _ in range(nv)]
+    for t, tri in enumerate(np.asarray(faces)):
+        for v in tri:
+            inc[v].append(t)
+    K = max(1, max(len(x) for x in inc))
+    tab = np.zeros((nv, K), np.int64)
+    mask = np.zeros((nv, K), np.float32)
+    for v, lst in enumerate(inc):
+        tab[v, :len(lst)] = lst
+        mask[v, :len(lst)] = 1.0
+    return tab, mask
+
+
+class Features:
+    """Per-vertex feature builder for one mesh topology (all torch gathers: deterministic)."""
+
+    def __init__(self, faces_np: np.ndarray, nv: int, lap, device):
+        tab, mask = vertex_face_table(faces_np, nv)
+        self.faces = torch.tensor(np.asarray(faces_np), device=device)
+        self.vf = torch.tensor(tab, device=device)
+        self.vf_mask = torch.tensor(mask, device=device)
+        self.lap = lap
+
+    @torch.no_grad()
+    def __call__(self, V: torch.Tensor, grad: torch.Tensor) -> torch.Tensor:
+        c = V.mean(0, keepdim=True)
+        P = V - c
+        rad = P.pow(2).sum(1).mean().sqrt().clamp_min(1e-12)
+        a, b, cc = V[self.faces[:, 0]], V[self.faces[:, 1]], V[self.faces[:, 2]]
+        fn = torch.cross(b - a, cc - a, dim=1)  # area-weighted face normals
+        vn = (fn[self.vf] * self.vf_mask[..., None]).sum(1)
+        vn = vn / vn.norm(dim=1, keepdim=True).clamp_min(1e-20)
+        gs = grad.pow(2).sum(1).mean().sqrt().clamp_min(1e-30)
+        g = grad / gs
+        lap = self.lap
+        gm = (g[lap.nbr] * lap.mask[..., None]).sum(1) * lap.invdeg[:, None]
+        return torch.cat([P / rad, vn, g, gm], dim=1).contiguous()
+
+
+class ProposalMLP:
+    """The proposal network: fp32 master parameters, bf16 weight copy, Adam state."""
+
+    def __init__(self, device, seed: int = 0, out_scale: float = 0.02):
+        L = _lib.lib()
+        self.nw = int(L.dmic_mlp_weight_count())
+        self.nb = int(L.dmic_mlp_bias_count())
+        self.device = torch.device(device)
+        self.out_scale = out_scale
+        g = torch.Generator().manual_seed(seed)
+        chunks = []
+        for fan_in, fan_out in ((N_IN, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_OUT)):
+            std = math.sqrt(2.0 / fan_in) if fan_out != N_OUT else 0.1 / math.sqrt(fan_in)
+            chunks.append(torch.randn(fan_in * fan_out, generator=g) * std)
+        self.w = torch.cat(chunks).to(self.device).contiguous()
+        self.b = torch.zeros(self.nb, device=self.device)
+        self.wb = torch.empty(self.nw, dtype=torch.int16, device=self.device)
+        self._sync_bf16()
+        self.mw = torch.zeros_like(self.w)
+        self.vw = torch.zeros_like(self.w)
+        self.mb = torch.zeros_like(self.b)
+        self.vb = torch.zeros_like(self.b)
+        self.step_count = 0
+
+    def _sync_bf16(self):
+        L = _lib.lib()
+        _lib.check(L.dmic_to_bf16(self.w.data_ptr(), self.nw, self.wb.data_ptr(),
+                                  _lib.stream_of(self.w)), "dmic_to_bf16")
+
+    def state_dict(self):
+        return {"w": self.w.cpu(), "b": self.b.cpu(), "out_scale": self.out_scale,
+                "step": self.step_count}
+
+    def load_state_dict(self, sd):
+        self.w = sd["w"].to(self.device).contiguous()
+        self.b = sd["b"].to(self.device).contiguous()
+        self.out_scale = float(sd["out_scale"])
+        self.step_count = int(sd.get("step", 0))
+        self._sync_bf16()
+
+    def forward(self, x: torch.Tensor, keep_acts: bool = False):
+        """Raw network output [rows, 3] (and hidden activations if keep_acts)."""
+        x = x.contiguous()
+        rows = x.shape[0]
+        y = torch.empty((rows, N_OUT), device=self.device)
+        acts = torch.empty((3, rows, N_HID), device=self.device) if keep_acts else None
+        scratch = None if keep_acts else torch.empty((2, rows, N_HID), device=self.device)
+        L = _lib.lib()
+        _lib.check(L.dmic_mlp_forward(self.wb.data_ptr(), self.b.data_ptr(), x.data_ptr(), rows,
+                                      y.data_ptr(), _lib.ptr(acts), _lib.ptr(scratch),
+                                      _lib.stream_of(y)), "dmic_mlp_forward")
+        return (y, acts) if keep_acts else y
+
+    def propose(self, x: torch.Tensor) -> torch.Tensor:
+        return self.out_scale * self.forward(x)
+
+    def backward(self, x, acts, dy):
+        rows = x.shape[0]
+        gw = torch.empty_like(self.w)
+        gb = torch.empty_like(self.b)
+        scratch = torch.empty((2, rows, N_HID), device=self.device)
+        L = _lib.lib()
+        _lib.check(L.dmic_mlp_backward(self.wb.data_ptr(), x.data_ptr(), acts.data_ptr(),
+                                       dy.contiguous().data_ptr(), rows, gw.data_ptr(),
+                                       gb.data_ptr(), scratch.data_ptr(), _lib.stream_of(gw)),
+                   "dmic_mlp_backward")
+        return gw, gb
+
+    def adam(self, gw, gb, lr=1e-3, betas=(0.9, 0.999), eps=1e-8):
+        self.step_count += 1
+        L = _lib.lib()
+        st = _lib.stream_of(self.w)
+        _lib.check(L.dmic_adam_step(self.w.data_ptr(), self.wb.data_ptr(), gw.data_ptr(),
+                                    self.mw.data_ptr(), self.vw.data_ptr(), self.nw, lr, betas[0],
+                                    betas[1], eps, self.step_count, st), "dmic_adam_step")
+        _lib.check(L.dmic_adam_step(self.b.data_ptr(), None, gb.data_ptr(), self.mb.data_ptr(),
+                                    self.vb.data_ptr(), self.nb, lr, betas[0], betas[1], eps,
+                                    self.step_count, st), "dmic_adam_step")
+
+    def train_step(self, x: torch.Tensor, target: torch.Tensor, lr=1e-3) -> float:
+        """One Adam step on mean((net(x) - target/out_scale)^2)."""
+        y, acts = self.forward(x, keep_acts=True)
+        t = target / self.out_scale
+        diff = y - t
+        loss = diff.pow(2).mean()
+        dy = (2.0 / diff.numel()) * diff
+        gw, gb = self.backward(x, acts, dy)
+        self.adam(gw, gb, lr=lr)
+        return loss
+
+
+# ------------------------------------------------------------------ training data
+@dataclass
+class TrajectoryData:
+    x: torch.Tensor        # [rows, 12] features
+    target: torch.Tensor   # [rows, 3] displacement over the horizon
+
+
+def trajectory_samples(problem: FitProblem, feats: Features, steps: int, every: int,
+                       horizon: int, lr: float = 1e-3) -> TrajectoryData:
+    """Runs a plain Adam fit and turns its snapshots into (features, displacement) pairs."""
+    from .fit import fit_adam
+    res = fit_adam(problem, steps=steps, lr=lr, snapshot_every=every, sync_every=0)
+    snaps = res.snapshots + [(steps, res.verts, None)]
+    by_step = {k: v for k, v, _ in snaps}
+    xs, ts = [], []
+    for k, V, g in res.snapshots:
+        k2 = min(k + horizon, steps)
+        if k2 == k or k2 not in by_step:
+            continue
+        xs.append(feats(V, g))
+        ts.append(by_step[k2] - V)
+    return TrajectoryData(torch.cat(xs), torch.cat(ts))
+
+
+# ------------------------------------------------------------------ learned fit
+def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: int = 1000,
+                lr: float = 1e-3, stop_loss: float | None = None,
+                scales=(1.0, 0.5, 0.25)) -> FitResult:
+    """Accept/reject fit: at each iteration the network proposes V + s*d for
+    the step scales ``scales`` (largest first); the first proposal whose loss
+    is below the current loss is accepted, otherwise a plain Adam step is
+    taken with the gradient already computed.  ``losses[k]`` is the loss of
+    the iterate after iteration k."""
+    V = problem.init_verts.clone()
+    Vp = V.clone().requires_grad_(True)
+    opt = torch.optim.Adam([Vp], lr=lr)
+    losses, times = [], []
+    accepted = proposals = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L, g = problem.loss_and_grad(Vp)
+    for k in range(steps):
+        x = feats(Vp.detach(), g)
+        d = net.propose(x)
+        took = False
+        for s in scales:
+            proposals += 1
+            with torch.no_grad():
+                Lc = problem.loss(Vp.detach() + s * d)
+            if float(Lc) < float(L):
+                with torch.no_grad():
+                    Vp.add_(s * d)
+                accepted += 1
+                took = True
+                break
+        if not took:
+            Vp.grad = g
+            opt.step()
+        L, g = problem.loss_and_grad(Vp)
+        losses.append(float(L))
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        if stop_loss is not None and losses[-1] <= stop_loss:
+            break
+    return FitResult(verts=Vp.detach(), losses=losses, times=times, accepted=accepted,
+                     proposals=proposals)

@@ -137,6 +137,47 @@ int dmic_gaussian_otf(const float* sigma_host, const float* xi0, const float* xi
                       const float* xi2, int64_t n0, int64_t n1, int64_t n2, float* out,
                       hipStream_t stream);
 
+/* ------------------------------------------------- proposal network (MLP)
+ *
+ * No counterpart in the reference: the learned per-vertex displacement
+ * proposal (12 -> 128 -> 128 -> 128 -> 3, ReLU) of the fitting loop.  Weights
+ * are bf16 (packed row-major [in][out]: W1 12x128, W2 128x128, W3 128x128,
+ * W4 128x3), biases fp32 (b1, b2, b3: 128, b4: 3); products accumulate in
+ * FP32 on the matrix cores (v_mfma_f32_16x16x16_bf16).
+ */
+
+/* C[M,N] = A(M,K) B(K,N) on MFMA with bf16 inputs and FP32 accumulation;
+ * A(i,k) = A[i*sa0 + k*sa1], B(k,j) = B[k*sb0 + j*sb1] (fp32 or bf16 storage).
+ * epilogue: 0 none, 1 + bias[j], 2 relu(+ bias[j]), 3 zero where mask[i*ldm+j] <= 0. */
+int dmic_gemm_bf16(const void* A, int a_is_bf16, int64_t sa0, int64_t sa1, const void* B,
+                   int b_is_bf16, int64_t sb0, int64_t sb1, float* C, int64_t ldc, int64_t M,
+                   int64_t N, int64_t K, const float* bias, const float* mask, int64_t ldm,
+                   int epilogue, hipStream_t stream);
+
+int64_t dmic_mlp_weight_count(void);
+int64_t dmic_mlp_bias_count(void);
+
+/* y[rows][3] = MLP(x[rows][12]).  acts (optional, [3][rows][128]) keeps the
+ * hidden activations for dmic_mlp_backward; without it scratch must hold
+ * 2*rows*128 floats. */
+int dmic_mlp_forward(const uint16_t* w_bf16, const float* bias, const float* x, int64_t rows,
+                     float* y, float* acts, float* scratch, hipStream_t stream);
+
+/* Gradients of sum(dy . y) w.r.t. the weights (grad_w, fp32) and biases
+ * (grad_b); scratch holds 2*rows*128 floats.  Deterministic. */
+int dmic_mlp_backward(const uint16_t* w_bf16, const float* x, const float* acts, const float* dy,
+                      int64_t rows, float* grad_w, float* grad_b, float* scratch,
+                      hipStream_t stream);
+
+/* Bias-corrected Adam on fp32 master parameters; w_bf16 (optional) receives
+ * the updated bf16 copy. */
+int dmic_adam_step(float* w, uint16_t* w_bf16, const float* g, float* m, float* v, int64_t n,
+                   float lr, float beta1, float beta2, float eps, int64_t step,
+                   hipStream_t stream);
+
+/* y = bf16(x), round to nearest even. */
+int dmic_to_bf16(const float* x, int64_t n, uint16_t* y, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,7 @@
 //
 //  1. "accurate" (orc_acc_*): the same mathematics evaluated to ~1e-12 in
 //     FP64, with a centroid Taylor series when all three phases of a face
-//     are close and an __float128 closed form when one pair of phases is
+//     are close and an __float128 evaluation when one pair of phases is
 //     nearly coincident.  Every error bound of the HIP path is stated against
 //     this oracle.  It is pinned by tests/test_oracle.py against mpmath
 //     quadrature of single triangles, the closed-form transform of a cube
@@ -127,44 +127,59 @@ void face_accurate(double xa, double xb, double xc, cd* F, cd* dF /*[3] or null*
     }
     return;
   }
-  // One pair of phases nearly coincident, the third >= 0.5 rad away:
-  // evaluate in __float128 from the same double phases.
-  const __float128 qa = xa, qb = xb, qc = xc;
-  const __float128 qab = qb - qa, qbc = qc - qb, qca = qa - qc;
-  if (mn > 1e-15) {
-    const Quad ea = qexpmi(qa), eb = qexpmi(qb), ec = qexpmi(qc);
-    const __float128 rab = 1 / qab, rbc = 1 / qbc, rca = 1 / qca;
-    Quad Fq = qadd(qadd(qscale(ea, rab * rca), qscale(eb, rbc * rab)), qscale(ec, rca * rbc));
-    *F = q2d(Fq);
-    if (dF) {
-      // dF_a = (rab - rca - i) rca rab ea + rab^2 rbc eb - rca^2 rbc ec
-      Quad t0 = qmul(Quad{rab - rca, -1}, qscale(ea, rca * rab));
-      Quad da = qsub(qadd(t0, qscale(eb, rab * rab * rbc)), qscale(ec, rca * rca * rbc));
-      Quad t1 = qmul(Quad{rbc - rab, -1}, qscale(eb, rab * rbc));
-      Quad db = qsub(qadd(t1, qscale(ec, rbc * rbc * rca)), qscale(ea, rab * rab * rca));
-      Quad t2 = qmul(Quad{rca - rbc, -1}, qscale(ec, rbc * rca));
-      Quad dc = qsub(qadd(t2, qscale(ea, rca * rca * rab)), qscale(eb, rbc * rbc * rab));
-      dF[0] = q2d(da); dF[1] = q2d(db); dF[2] = q2d(dc);
+  // One pair of phases within 1e-3 rad, the largest gap D >= 1 rad.  The
+  // closed form cancels like eps/d^2 here, so evaluate in __float128 with
+  // the phases re-based on the median vertex (phases lo <= mid <= hi,
+  // P = lo - mid, Q = hi - mid, D = Q - P):
+  //   F = e^{-i mid} (phi(P) - phi(Q)) / D,  phi(y) = (e^{-iy} - 1)/y,
+  //   dF/dlo = e^{-i mid} (phi'(P) + H)/D,  dF/dhi = -e^{-i mid} (phi'(Q) + H)/D,
+  //   dF/dmid = -iF - dF/dlo - dF/dhi,
+  // where phi(y) = -(sin(y/2)/(y/2)) (sin(y/2) + i cos(y/2)) has no
+  // cancellation and phi'(y) uses its Taylor series for |y| < 1/4.
+  // Pinned by the quadrature tests (nearly coincident pairs).
+  const __float128 xs[3] = {(__float128)xa, (__float128)xb, (__float128)xc};
+  int ord[3] = {0, 1, 2};
+  std::sort(ord, ord + 3, [&](int i, int j) { return xs[i] < xs[j] || (xs[i] == xs[j] && i < j); });
+  const __float128 mid = xs[ord[1]];
+  const __float128 P = xs[ord[0]] - mid, Q = xs[ord[2]] - mid, D = Q - P;
+  auto phi = [](__float128 y) -> Quad {
+    const __float128 h = y / 2, sh = sinq(h), ch = cosq(h);
+    const __float128 k = (h == 0) ? (__float128)1 : sh / h;
+    return {-k * sh, -k * ch};
+  };
+  auto dphi = [&](__float128 y) -> Quad {
+    if (fabsq(y) >= (__float128)0.25) {
+      const Quad e = qexpmi(y);          // e^{-iy}
+      const Quad mie = {e.im, -e.re};    // -i e^{-iy}
+      return qscale(qsub(mie, phi(y)), 1 / y);
     }
-    return;
-  }
-  // |pair| <= 1e-15: collapse the pair onto its midpoint (exact to O(h^2)
-  // for F and O(h) ~ 1e-15 for the derivatives), as in .cu:254-271 and
-  // .cu:94-117, in __float128.
-  int odd;  // index of the unpaired vertex
-  if (aab == mn) odd = 2; else if (abc == mn) odd = 0; else odd = 1;
-  const __float128 xs[3] = {qa, qb, qc};
-  const __float128 mq = (xs[(odd + 1) % 3] + xs[(odd + 2) % 3]) / 2;
-  const __float128 dq = mq - xs[odd];
-  const Quad emq = qexpmi(mq), eo = qexpmi(xs[odd]);
-  const Quad iem = {-emq.im, emq.re};  // i*e_m
-  Quad Fq = qscale(qadd(iem, qscale(qsub(emq, eo), 1 / dq)), 1 / dq);
+    // sum_{k>=2} (-i)^k (k-1) y^{k-2} / k!
+    Quad sum = {0, 0};
+    __float128 yp = 1, fact = 2;
+    for (int k = 2; k <= 34; ++k) {
+      if (k > 2) { yp *= y; fact *= k; }
+      const __float128 c = (k - 1) * yp / fact;
+      switch (k & 3) {
+        case 0: sum.re += c; break;
+        case 1: sum.im -= c; break;
+        case 2: sum.re -= c; break;
+        case 3: sum.im += c; break;
+      }
+    }
+    return sum;
+  };
+  const Quad em = qexpmi(mid);
+  const Quad H = qscale(qsub(phi(P), phi(Q)), 1 / D);
+  const Quad Fq = qmul(em, H);
   *F = q2d(Fq);
   if (dF) {
-    Quad g = qscale(qsub(qscale(emq, (__float128)0.5), Fq), 1 / dq);
-    Quad miF = {Fq.im, -Fq.re};  // -i F
-    Quad godd = qsub(miF, qscale(g, 2));
-    for (int j = 0; j < 3; ++j) dF[j] = q2d(j == odd ? godd : g);
+    const Quad dlo = qmul(em, qscale(qadd(dphi(P), H), 1 / D));
+    const Quad dhi = qmul(em, qscale(qadd(dphi(Q), H), -1 / D));
+    const Quad miF = {Fq.im, -Fq.re};
+    const Quad dmid = qsub(qsub(miF, dlo), dhi);
+    dF[ord[0]] = q2d(dlo);
+    dF[ord[1]] = q2d(dmid);
+    dF[ord[2]] = q2d(dhi);
   }
 }
 

@@ -91,12 +91,20 @@ def face_ref(xa, xb, xc, dtype=np.float64):
 
 
 # --------------------------------------------------------------- mesh level
-def acc_forward(verts, faces, coeff, xi, threads=None):
-    """Accurate ftmesh_list (complex128 [n]) on exactly the given FP32 inputs."""
-    V = _c(np.asarray(verts, np.float32), np.float64)
+def _inputs(verts, coeff, xi, fp32_inputs):
+    if fp32_inputs:  # the values the HIP path receives, widened exactly
+        verts = np.asarray(verts, np.float32)
+        coeff = np.asarray(coeff, np.float32)
+        xi = np.asarray(xi, np.float32)
+    return (_c(verts, np.float64), _c(coeff, np.float64), _c(xi, np.float64).reshape(-1, 3))
+
+
+def acc_forward(verts, faces, coeff, xi, threads=None, fp32_inputs=True):
+    """Accurate ftmesh_list (complex128 [n]).  By default the inputs are first
+    rounded to FP32 (what the HIP path sees); fp32_inputs=False keeps FP64
+    inputs (finite differences)."""
+    V, C, X = _inputs(verts, coeff, xi, fp32_inputs)
     F = _c(faces, np.int64)
-    C = _c(np.asarray(coeff, np.float32), np.float64)
-    X = _c(np.asarray(xi, np.float32), np.float64).reshape(-1, 3)
     out = np.zeros((len(X), 2))
     rc = lib().orc_acc_forward(_p(V, _d), len(V), _p(F, _l), len(F), _p(C, _d), _p(X, _d), len(X),
                                _p(out, _d), _threads(threads))
@@ -104,12 +112,10 @@ def acc_forward(verts, faces, coeff, xi, threads=None):
     return out[:, 0] + 1j * out[:, 1]
 
 
-def acc_backward(verts, faces, coeff, xi, grad, threads=None, return_face=False):
+def acc_backward(verts, faces, coeff, xi, grad, threads=None, return_face=False, fp32_inputs=True):
     """Accurate d/dV sum Re(conj(grad) ftmesh) (float64 [nv, 3])."""
-    V = _c(np.asarray(verts, np.float32), np.float64)
+    V, C, X = _inputs(verts, coeff, xi, fp32_inputs)
     F = _c(faces, np.int64)
-    C = _c(np.asarray(coeff, np.float32), np.float64)
-    X = _c(np.asarray(xi, np.float32), np.float64).reshape(-1, 3)
     g = np.asarray(grad)
     G = _c(np.stack([g.real, g.imag], axis=1), np.float64)
     out = np.zeros((len(V), 3))

@@ -94,7 +94,9 @@ def test_forward_cfg2_subset(device):
     w = syn.config2()
     bs, axes, otf, band = _band(w, device)
     n = band.idx.numel()
-    assert 780000 < n < 850000
+    # (4/3) pi (r_band / (2 pi / L))^3 with r_band = sqrt(2 ln(1e5) / 1e-3) and
+    # this shape's box edge L ~ 2.64: ~1.08e6 band frequencies
+    assert 1.0e6 < n < 1.2e6
     sel = np.sort(syn.rng(w.seed + 1).choice(n, 4096, replace=False))
     xi_sub = band.xi_list[torch.tensor(sel, device=device)].contiguous()
     Vs = _shifted(w, bs)
@@ -163,3 +165,38 @@ def test_bitwise_deterministic(device):
     ga = backward_fourier3d_cpp(V, Fc, C, g, a, band.xi_list)
     gb = backward_fourier3d_cpp(V, Fc, C, g, a, band.xi_list)
     assert torch.equal(ga, gb)
+
+
+def test_precision_stress_cfg3_needles(device):
+    """Config 3: 2% near-degenerate faces (one edge 1e-4 of its length),
+    anisotropic OTF, threshold 1e-7, 256 x 256 x 128.  Errors on 4096 seeded
+    band frequencies, for the whole mesh and separately for the needle faces
+    alone (Faces_coeff = 1 on needles, 0 elsewhere)."""
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    w = syn.config3()
+    bs, axes, otf, band = _band(w, device)
+    n = band.idx.numel()
+    assert 6.0e5 < n < 1.2e6
+    sel = np.sort(syn.rng(w.seed + 1).choice(n, 4096, replace=False))
+    xi_sub = band.xi_list[torch.tensor(sel, device=device)].contiguous()
+    xi_np = xi_sub.cpu().numpy()
+    Vs = _shifted(w, bs)
+    V = torch.tensor(Vs, device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    g = _grad(4096, w.seed + 4)
+    gt = torch.tensor(g, device=device)
+    results = {}
+    needle_coeff = np.zeros(len(w.faces), np.float32)
+    needle_coeff[w.extra["needle_faces"]] = 1.0
+    for label, coeff in (("all faces", w.coeff), ("needle faces only", needle_coeff)):
+        C = torch.tensor(coeff, device=device)
+        ft = forward_fourier3d_cpp(V, Fc, C, xi_sub)
+        gv = backward_fourier3d_cpp(V, Fc, C, gt, ft, xi_sub)
+        ref = O.acc_forward(Vs, w.faces, coeff, xi_np, threads=threads())
+        refg = O.acc_backward(Vs, w.faces, coeff, xi_np, g, threads=threads())
+        results[label] = (O.rel_l2(ft.cpu().numpy(), ref), O.rel_l2(gv.cpu().numpy(), refg))
+        print(f"cfg3 {label}: forward rel-L2 {results[label][0]:.3e}, adjoint rel-L2 "
+              f"{results[label][1]:.3e}")
+    for fe, ge in results.values():
+        assert fe <= FWD_TOL
+        assert ge <= BWD_TOL

@@ -1,0 +1,187 @@
+"""CPU-side checks: the C-ABI library loads and exports its header, the host
+logic (generators, box, grid, API errors) behaves like the reference's."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from deltamic_amd import synthetic as syn
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "deltamic.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dmic_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from deltamic_amd import _lib
+    L = _lib.lib()
+    names = _header_functions()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in _lib.SIGNATURES, f"{n} has no ctypes signature"
+    assert L.dmic_version() >= 100
+    nm = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                        text=True, check=True).stdout
+    exported = set(re.findall(r" T (dmic_\w+)", nm))
+    assert exported == set(names)
+
+
+def test_workspace_sizes_are_sane():
+    from deltamic_amd import _lib
+    L = _lib.lib()
+    assert L.dmic_meshft_forward_workspace(10242, 20480, 815400) < 64 << 20
+    assert L.dmic_meshft_backward_workspace(10242, 20480, 815400) < 256 << 20
+    assert L.dmic_meshft_forward_workspace(642, 1280, 22000) > 0
+    assert L.dmic_band_workspace(200, 200, 200) > 0
+    assert L.dmic_meshft_forward_workspace(0, 0, 0) > 0
+
+
+def test_icosphere_counts_closed_and_outward():
+    for s, (nv, nf) in {0: (12, 20), 3: (642, 1280), 4: (2562, 5120), 5: (10242, 20480)}.items():
+        v, f = syn.icosphere(s)
+        assert v.shape == (nv, 3) and f.shape == (nf, 3)
+        assert np.allclose(np.linalg.norm(v, axis=1), 1.0)
+        edges = {}
+        for a, b, c in f:
+            for e in ((a, b), (b, c), (c, a)):
+                edges[e] = edges.get(e, 0) + 1
+        # every directed edge once and its reverse once: closed, consistently oriented
+        assert all(cnt == 1 for cnt in edges.values())
+        assert all((b, a) in edges for (a, b) in edges)
+        assert nv - len(edges) // 2 + nf == 2
+        p0, p1, p2 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+        vol = np.einsum("ij,ij->i", p0, np.cross(p1, p2)).sum() / 6
+        assert vol > 0
+
+
+def test_real_sph_harm_orthonormal():
+    u, f = syn.icosphere(5)
+    p0, p1, p2 = u[f[:, 0]], u[f[:, 1]], u[f[:, 2]]
+    c = (p0 + p1 + p2)
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    area = 0.5 * np.linalg.norm(np.cross(p1 - p0, p2 - p0), axis=1)
+    area *= 4 * np.pi / area.sum()
+    th = np.arccos(np.clip(c[:, 2], -1, 1))
+    ph = np.arctan2(c[:, 1], c[:, 0])
+    Y = np.stack([syn.real_sph_harm(l, m, th, ph) for l in range(4) for m in range(-l, l + 1)])
+    G = (Y * area) @ Y.T
+    assert np.allclose(G, np.eye(len(Y)), atol=2e-3)
+
+
+def test_workload_generators():
+    w1 = syn.config1()
+    assert w1.verts.shape == (642, 3) and w1.faces.shape == (1280, 3)
+    assert w1.verts.dtype == np.float32
+    assert np.array_equal(syn.config1().verts, w1.verts)  # seeded
+    r = np.linalg.norm(w1.verts - w1.verts.mean(0), axis=1)
+    assert abs(r.mean() - 0.3) < 0.01
+    w2 = syn.config2()
+    assert w2.verts.shape == (10242, 3) and w2.faces.shape == (20480, 3)
+    L = w2.box_size[0, 1] - w2.box_size[0, 0]
+    assert 2.2 < L < 2.9
+    assert np.allclose(w2.box_size[:, 0], w2.box_size[0, 0])  # cubic box
+    w3 = syn.config3()
+    nd = w3.extra["needle_faces"]
+    assert len(nd) == 410
+    p0, p1, p2 = (w3.verts[w3.faces[nd, j]].astype(np.float64) for j in range(3))
+    shortest = np.min(np.stack([np.linalg.norm(p1 - p0, axis=1), np.linalg.norm(p2 - p1, axis=1),
+                                np.linalg.norm(p0 - p2, axis=1)]), axis=0)
+    assert shortest.max() < 1e-5
+    assert w3.box_shape == (256, 256, 128)
+    w4 = syn.config4()
+    assert w4.faces.shape == (5120, 3) and w4.extra["init_verts"].shape == (2562, 3)
+
+
+def test_compute_box_size_semantics():
+    from deltamic_amd import compute_box_size
+    v = np.array([[0.0, 0.0, 0.0], [1.0, 2.0, 4.0]])
+    b = compute_box_size(v, offset=0.5)
+    # per axis: [min - ext*0.25, max + ext*0.25] -> x [-0.25,1.25], y [-0.5,2.5], z [-1,5]
+    # then every axis takes the overall min and max (render.py:16-17)
+    assert np.allclose(b, [[-1.0, 5.0]] * 3)
+    assert np.allclose(compute_box_size(v), [[-0.2, 4.2]] * 3)  # default offset 0.1
+
+
+def test_frequency_axes_bit_identical_to_restatement():
+    from deltamic_amd import Fourier3dMesh
+    w = syn.config3()
+    m = Fourier3dMesh(torch.tensor(w.box_size), w.box_shape, device="cpu", dtype=torch.float32)
+    ax = O.frequency_axes(m.box_size.numpy(), w.box_shape)
+    for a, b in zip((m.xi0, m.xi1, m.xi2), ax):
+        assert a.dtype == torch.float32
+        assert np.array_equal(a.numpy(), b)
+    # no zero bin on even grids, symmetric about zero
+    assert np.array_equal(m.xi0.numpy(), -m.xi0.numpy()[::-1])
+    assert m.OTF.shape == tuple(w.box_shape) and bool((m.OTF == 1).all())
+    assert m.narrowband_thresh == 0.01
+
+
+def test_cpu_device_raises_instead_of_falling_back():
+    from deltamic_amd import Fourier3dMesh
+    w = syn.config1()
+    m = Fourier3dMesh(torch.tensor(w.box_size), (8, 8, 8), device="cpu")
+    with pytest.raises(NotImplementedError):
+        m(torch.tensor(w.verts), torch.tensor(w.faces), torch.ones(len(w.faces)))
+
+
+def test_native_boundary_checks_like_reference():
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    V = torch.zeros(3, 3)
+    F = torch.tensor([[0, 1, 2]])
+    C = torch.ones(1)
+    xi = torch.zeros(4, 3)
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        forward_fourier3d_cpp(V, F, C, xi)
+    with pytest.raises(RuntimeError, match="must be a CUDA tensor"):
+        backward_fourier3d_cpp(V, F, C, torch.zeros(4, dtype=torch.complex64),
+                               torch.zeros(4, dtype=torch.complex64), xi)
+
+
+def test_gaussian_psf_cpu_matches_restatement():
+    from deltamic_amd import generate_gaussian_psf
+    ax = [torch.linspace(-30, 30, n) for n in (5, 6, 7)]
+    S = torch.tensor([[1e-3, 2e-4, 0.0], [2e-4, 2e-3, 1e-4], [0.0, 1e-4, 4e-3]])
+    got = generate_gaussian_psf(S, *ax).numpy()
+    want = O.gaussian_otf(S.numpy(), *[a.numpy() for a in ax])
+    assert np.allclose(got, want, rtol=1e-5)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout
+
+
+def test_uniform_laplacian_adjoint_matches_dense_autograd():
+    """The padded-neighbour Laplacian (deterministic adjoint) equals the dense
+    (I - W) formulation, value and gradient."""
+    from deltamic_amd.fit import UniformLaplacian, neighbour_table
+    u, f = syn.icosphere(2)
+    V = torch.tensor(u + syn.rng(5).normal(0, 0.05, u.shape), dtype=torch.float64,
+                     requires_grad=True)
+    lap = UniformLaplacian(f, len(u), "cpu")
+    tab, deg = neighbour_table(f, len(u))
+    lap.mask = lap.mask.double()
+    lap.invdeg = torch.tensor(1.0 / deg, dtype=torch.float64)
+    val = lap(V)
+    (g,) = torch.autograd.grad(val, V)
+    W = np.zeros((len(u), len(u)))
+    for v in range(len(u)):
+        W[v, tab[v, :deg[v]]] = 1.0 / deg[v]
+    M = torch.tensor(np.eye(len(u)) - W)
+    V2 = V.detach().clone().requires_grad_(True)
+    val2 = ((M @ V2) ** 2).sum(1).mean()
+    (g2,) = torch.autograd.grad(val2, V2)
+    assert abs(float(val) - float(val2)) <= 1e-12
+    assert torch.allclose(g, g2, atol=1e-12)
