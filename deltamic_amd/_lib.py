@@ -38,6 +38,9 @@ SIGNATURES = {
     "dmic_band_fill": ([_vp, _i64, _i64, _i64, _f32, _vp, _vp, _vp, _vp, _vp, _vp, _sz, _vp], _int),
     "dmic_band_scatter": ([_vp, _vp, _i64, _vp, _i64, _vp], _int),
     "dmic_band_gather": ([_vp, _vp, _i64, _vp, _vp], _int),
+    "dmic_band_symmetry": ([_vp, _i64, _i64, _i64, _f32, _vp, _vp, _vp, _vp, _vp], _int),
+    "dmic_band_mirror_fill": ([_vp, _i64, _vp], _int),
+    "dmic_band_mirror_fold": ([_vp, _i64, _vp, _vp], _int),
     "dmic_gaussian_otf": ([_vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp], _int),
     "dmic_gemm_bf16": ([_vp, _int, _i64, _i64, _vp, _int, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
                         _vp, _vp, _i64, _int, _vp], _int),

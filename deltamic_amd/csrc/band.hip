@@ -128,6 +128,50 @@ __global__ __launch_bounds__(256) void band_gather_kernel(const float2* __restri
   if (j < n) vals[j] = box[idx[j]];
 }
 
+// Centrosymmetry of the band: flag[0] |= 1 if (f[i] > t) != (f[N-1-i] > t)
+// for some voxel, |= 2 if an axis is not exactly antisymmetric
+// (xi_d[k] != -xi_d[n_d-1-k]).
+__global__ __launch_bounds__(256) void band_symmetry_kernel(const float* __restrict__ filter,
+                                                            int64_t nvox, float thresh,
+                                                            const float* __restrict__ xi0, int64_t n0,
+                                                            const float* __restrict__ xi1, int64_t n1,
+                                                            const float* __restrict__ xi2, int64_t n2,
+                                                            int* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int bad = 0;
+  if (i < nvox / 2) bad |= ((filter[i] > thresh) != (filter[nvox - 1 - i] > thresh)) ? 1 : 0;
+  if (i < n0 && xi0[i] != -xi0[n0 - 1 - i]) bad |= 2;
+  if (i < n1 && xi1[i] != -xi1[n1 - 1 - i]) bad |= 2;
+  if (i < n2 && xi2[i] != -xi2[n2 - 1 - i]) bad |= 2;
+  if (bad) atomicOr(flag, bad);
+}
+
+// vals[n-1-j] = conj(vals[j]) for j < n/2 (the first ceil(n/2) entries are given).
+__global__ __launch_bounds__(256) void band_mirror_fill_kernel(float2* __restrict__ vals,
+                                                               int64_t n) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= n / 2) return;
+  const float2 v = vals[j];
+  vals[n - 1 - j] = make_float2(v.x, -v.y);
+}
+
+// g_half[j] = g[j] + conj(g[n-1-j]) for j < n/2; the self-mirrored middle
+// entry (n odd) is copied.
+__global__ __launch_bounds__(256) void band_mirror_fold_kernel(const float2* __restrict__ g,
+                                                               int64_t n,
+                                                               float2* __restrict__ g_half) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t h = (n + 1) / 2;
+  if (j >= h) return;
+  const float2 a = g[j];
+  if (j < n / 2) {
+    const float2 b = g[n - 1 - j];
+    g_half[j] = make_float2(a.x + b.x, a.y - b.y);
+  } else {
+    g_half[j] = a;
+  }
+}
+
 struct Sigma9 {
   float s[9];
 };
@@ -239,6 +283,45 @@ int dmic_band_gather(const float* box, const int32_t* idx, int64_t n, float* val
   DMIC_REQUIRE(box && idx && vals, DMIC_ERR_ARG, "null pointer");
   hipLaunchKernelGGL(band_gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, stream,
                      reinterpret_cast<const float2*>(box), idx, n, reinterpret_cast<float2*>(vals));
+  DMIC_LAUNCH_CHECK();
+  return DMIC_OK;
+}
+
+int dmic_band_symmetry(const float* filter, int64_t n0, int64_t n1, int64_t n2, float thresh,
+                       const float* xi0, const float* xi1, const float* xi2, int* flag_dev,
+                       hipStream_t stream) {
+  DMIC_REQUIRE(n0 >= 0 && n1 >= 0 && n2 >= 0, DMIC_ERR_ARG, "negative box shape");
+  DMIC_REQUIRE(flag_dev, DMIC_ERR_ARG, "null flag");
+  const int64_t nvox = n0 * n1 * n2;
+  int64_t m = nvox / 2;
+  if (n0 > m) m = n0;
+  if (n1 > m) m = n1;
+  if (n2 > m) m = n2;
+  if (m == 0) return DMIC_OK;
+  DMIC_REQUIRE(filter && xi0 && xi1 && xi2, DMIC_ERR_ARG, "null pointer");
+  hipLaunchKernelGGL(band_symmetry_kernel, dim3(ceil_div(m, 256)), dim3(256), 0, stream, filter,
+                     nvox, thresh, xi0, n0, xi1, n1, xi2, n2, flag_dev);
+  DMIC_LAUNCH_CHECK();
+  return DMIC_OK;
+}
+
+int dmic_band_mirror_fill(float* vals, int64_t n, hipStream_t stream) {
+  DMIC_REQUIRE(n >= 0, DMIC_ERR_ARG, "negative size");
+  if (n < 2) return DMIC_OK;
+  DMIC_REQUIRE(vals, DMIC_ERR_ARG, "null pointer");
+  hipLaunchKernelGGL(band_mirror_fill_kernel, dim3(ceil_div(n / 2, 256)), dim3(256), 0, stream,
+                     reinterpret_cast<float2*>(vals), n);
+  DMIC_LAUNCH_CHECK();
+  return DMIC_OK;
+}
+
+int dmic_band_mirror_fold(const float* g, int64_t n, float* g_half, hipStream_t stream) {
+  DMIC_REQUIRE(n >= 0, DMIC_ERR_ARG, "negative size");
+  if (n == 0) return DMIC_OK;
+  DMIC_REQUIRE(g && g_half, DMIC_ERR_ARG, "null pointer");
+  hipLaunchKernelGGL(band_mirror_fold_kernel, dim3(ceil_div((n + 1) / 2, 256)), dim3(256), 0,
+                     stream, reinterpret_cast<const float2*>(g), n,
+                     reinterpret_cast<float2*>(g_half));
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
 }

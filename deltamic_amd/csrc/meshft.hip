@@ -248,19 +248,20 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
         const cf em = expmi_phase(xa + (double)so.mid);
         cf H, dP, dQ;
         face_H_grad(so, H, dP, dQ);
-        const cf F = cmul(em, H);
-        const cf dlo = cmul(em, dP);
-        const cf dhi = cmul(em, dQ);
-        const cf dmid = {F.im - dlo.re - dhi.re, -F.re - dlo.im - dhi.im};
+        // Contract with conj(g) first: z = conj(g) e_mid, then
+        //   rho = Re(conj g F) = Re(z H),  gamma = Re(conj g (-iF)) = Im(z H),
+        //   Re(conj g dF/dx_lo) = Re(z dH/dP),  Re(conj g dF/dx_hi) = Re(z dH/dQ),
+        //   Re(conj g dF/dx_mid) = gamma - (lo) - (hi).
+        const cf z = cmul({fr.gr, -fr.gi}, em);
+        const float rh = z.re * H.re - z.im * H.im;
+        const float ga = z.re * H.im + z.im * H.re;
+        const float alo = z.re * dP.re - z.im * dP.im;
+        const float ahi = z.re * dQ.re - z.im * dQ.im;
+        const float amid = ga - alo - ahi;
         const int ra = (p < 0.0f) + (q < 0.0f);
         const int rb = (p >= 0.0f) + (q < p);
-        const cf dA = (ra == 0) ? dlo : ((ra == 2) ? dhi : dmid);
-        const cf dB = (rb == 0) ? dlo : ((rb == 2) ? dhi : dmid);
-        const float gr = fr.gr, gi = fr.gi;
-        const float al = __builtin_fmaf(gr, dA.re, gi * dA.im);
-        const float be = __builtin_fmaf(gr, dB.re, gi * dB.im);
-        const float ga = __builtin_fmaf(gr, F.im, -gi * F.re);
-        const float rh = __builtin_fmaf(gr, F.re, gi * F.im);
+        const float al = (ra == 0) ? alo : ((ra == 2) ? ahi : amid);
+        const float be = (rb == 0) ? alo : ((rb == 2) ? ahi : amid);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           a[d] = __builtin_fmaf(al, fr.x[d], a[d]);

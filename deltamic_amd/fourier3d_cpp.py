@@ -68,8 +68,12 @@ def validate_faces(mesh_faces: torch.Tensor, nv: int) -> None:
 
 
 def forward_fourier3d_cpp(mesh_vertices: torch.Tensor, mesh_faces: torch.Tensor,
-                          faces_coeff: torch.Tensor, xi_list: torch.Tensor) -> torch.Tensor:
-    """ftmesh_list[n] complex64 (fourier3d_cuda.cpp:27-42, .cu:421-462)."""
+                          faces_coeff: torch.Tensor, xi_list: torch.Tensor,
+                          out: torch.Tensor | None = None) -> torch.Tensor:
+    """ftmesh_list[n] complex64 (fourier3d_cuda.cpp:27-42, .cu:421-462).
+
+    ``out`` (optional, complex64, at least n entries) receives the values in
+    its first n entries."""
     _check_mesh(mesh_vertices, mesh_faces, faces_coeff)
     _check_input(xi_list, "xi_list")
     if xi_list.dtype != torch.float32 or xi_list.dim() != 2 or xi_list.shape[1] != 3:
@@ -77,7 +81,11 @@ def forward_fourier3d_cpp(mesh_vertices: torch.Tensor, mesh_faces: torch.Tensor,
     nv, nf, n = mesh_vertices.shape[0], mesh_faces.shape[0], xi_list.shape[0]
     validate_faces(mesh_faces, nv)
     dev = mesh_vertices.device
-    out = torch.empty((n,), dtype=torch.complex64, device=dev)
+    if out is None:
+        out = torch.empty((n,), dtype=torch.complex64, device=dev)
+    elif (out.dtype != torch.complex64 or not out.is_contiguous() or out.numel() < n
+          or out.device != dev):
+        raise ValueError("out must be a contiguous complex64 tensor with at least n entries")
     L = _lib.lib()
     nbytes = L.dmic_meshft_forward_workspace(nv, nf, n)
     ws = _lib.workspace(nbytes, dev)

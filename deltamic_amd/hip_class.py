@@ -10,7 +10,13 @@ work done by HIP kernels:
   tensor, its version counter, the threshold or the frequency axes change
   (the reference rebuilds it on every forward);
 * scatter into the full complex box (cuda_class.py:27-28) and the gather of
-  ``grad_output`` (cuda_class.py:36).
+  ``grad_output`` (cuda_class.py:36);
+* Hermitian halving: when the band is exactly centrosymmetric (always the
+  case for the reference's grid with an even OTF such as the Gaussian), band
+  entry n-1-j is the mirror -xi of entry j and ftmesh(-xi) = conj(ftmesh(xi)),
+  so the forward evaluates the first (n+1)/2 frequencies and mirrors the rest,
+  and the adjoint runs on the same half with the folded upstream gradient
+  g[j] + conj(g[n-1-j]).  ``HERMITIAN_HALVING = False`` turns it off.
 
 Unlike cuda_class.py:8 importing this module does not reseed torch's RNG.
 """
@@ -25,12 +31,24 @@ from . import _lib
 from .fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
 
 
+HERMITIAN_HALVING = True
+
+
 @dataclass
 class Band:
     idx: torch.Tensor       # int32 [n] linear voxel indices, row-major
     xi_list: torch.Tensor   # float32 [n, 3]
     shape: tuple
     key: tuple
+    symmetric: bool = False  # entry n-1-j is exactly -xi of entry j
+
+    @property
+    def n(self) -> int:
+        return self.idx.numel()
+
+    @property
+    def half(self) -> int:
+        return (self.n + 1) // 2
 
 
 def _thresh_value(narrowband_thresh) -> float:
@@ -66,12 +84,18 @@ def compute_band(Filter: torch.Tensor, thresh: float, xi0: torch.Tensor, xi1: to
     n = int(count.item())
     idx = torch.empty((n,), dtype=torch.int32, device=dev)
     xi_list = torch.empty((n, 3), dtype=torch.float32, device=dev)
+    symmetric = False
     if n:
         _lib.check(L.dmic_band_fill(f.data_ptr(), n0, n1, n2, float(thresh), axes[0].data_ptr(),
                                     axes[1].data_ptr(), axes[2].data_ptr(), idx.data_ptr(),
                                     xi_list.data_ptr(), ws.data_ptr(), ws.numel(), st),
                    "dmic_band_fill")
-    return Band(idx=idx, xi_list=xi_list, shape=(n0, n1, n2), key=())
+        flag = torch.zeros((1,), dtype=torch.int32, device=dev)
+        _lib.check(L.dmic_band_symmetry(f.data_ptr(), n0, n1, n2, float(thresh),
+                                        axes[0].data_ptr(), axes[1].data_ptr(), axes[2].data_ptr(),
+                                        flag.data_ptr(), st), "dmic_band_symmetry")
+        symmetric = int(flag.item()) == 0
+    return Band(idx=idx, xi_list=xi_list, shape=(n0, n1, n2), key=(), symmetric=symmetric)
 
 
 def band_for(Filter: torch.Tensor, narrowband_thresh, xi0, xi1, xi2) -> Band:
@@ -109,6 +133,30 @@ def gather_band(box: torch.Tensor, band: Band) -> torch.Tensor:
     return vals
 
 
+def band_forward(V, Fc, C, band: Band) -> torch.Tensor:
+    """ftmesh_list [n] complex64 on the band (half band + mirror when symmetric)."""
+    if not (HERMITIAN_HALVING and band.symmetric and band.n > 1):
+        return forward_fourier3d_cpp(V, Fc, C, band.xi_list)
+    out = torch.empty((band.n,), dtype=torch.complex64, device=V.device)
+    forward_fourier3d_cpp(V, Fc, C, band.xi_list[:band.half], out=out)
+    L = _lib.lib()
+    _lib.check(L.dmic_band_mirror_fill(out.data_ptr(), band.n, _lib.stream_of(out)),
+               "dmic_band_mirror_fill")
+    return out
+
+
+def band_backward(V, Fc, C, grad_list, ftmesh_list, band: Band) -> torch.Tensor:
+    """gradVerts [nv, 3] from the band upstream gradient (folded to the half band when symmetric)."""
+    if not (HERMITIAN_HALVING and band.symmetric and band.n > 1):
+        return backward_fourier3d_cpp(V, Fc, C, grad_list, ftmesh_list, band.xi_list)
+    g_half = torch.empty((band.half,), dtype=torch.complex64, device=V.device)
+    L = _lib.lib()
+    _lib.check(L.dmic_band_mirror_fold(grad_list.data_ptr(), band.n, g_half.data_ptr(),
+                                       _lib.stream_of(g_half)), "dmic_band_mirror_fold")
+    return backward_fourier3d_cpp(V, Fc, C, g_half, ftmesh_list[:band.half],
+                                  band.xi_list[:band.half])
+
+
 class MeshFTHIP(Function):
     """HIP-based Function for the Fourier transform of a triangle mesh."""
 
@@ -120,7 +168,7 @@ class MeshFTHIP(Function):
         V = Verts.detach().contiguous()
         Fc = Faces.contiguous()
         C = Faces_coeff.detach().to(torch.float32).contiguous()
-        ftmesh_list = forward_fourier3d_cpp(V, Fc, C, band.xi_list)
+        ftmesh_list = band_forward(V, Fc, C, band)
         ctx.save_for_backward(V, Fc, C, ftmesh_list)
         ctx.band = band
         ftmesh = scatter_band(ftmesh_list, band)
@@ -133,7 +181,7 @@ class MeshFTHIP(Function):
         V, Fc, C, ftmesh_list = ctx.saved_tensors
         band = ctx.band
         grad_list = gather_band(grad_output, band)
-        gradVerts = backward_fourier3d_cpp(V, Fc, C, grad_list, ftmesh_list, band.xi_list)
+        gradVerts = band_backward(V, Fc, C, grad_list, ftmesh_list, band)
         return gradVerts, None, None, None, None, None, None, None
 
 

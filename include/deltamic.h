@@ -129,6 +129,27 @@ int dmic_band_gather(const float* box, const int32_t* idx, int64_t n, float* val
                      hipStream_t stream);
 
 /*
+ * Hermitian halving (no counterpart in the reference, which evaluates every
+ * band frequency).  On the reference's grid the axes are exactly
+ * antisymmetric (xi_d[k] = -xi_d[n_d-1-k], render.py:163-171); if the band
+ * mask is centrosymmetric as well, band entry n-1-j is the mirror -xi of
+ * entry j and ftmesh(-xi) = conj(ftmesh(xi)) for a real mesh.
+ *
+ * dmic_band_symmetry ORs into *flag_dev (device int, caller zeroes it):
+ * 1 if the mask is not centrosymmetric, 2 if an axis is not antisymmetric.
+ * dmic_band_mirror_fill: vals[n-1-j] = conj(vals[j]) for j < n/2, after the
+ * forward has written the first (n+1)/2 entries.
+ * dmic_band_mirror_fold: g_half[j] = g[j] + conj(g[n-1-j]) for j < n/2 (the
+ * middle entry of an odd band is copied): the upstream gradient of the
+ * half band, so the adjoint runs on (n+1)/2 frequencies.
+ */
+int dmic_band_symmetry(const float* filter, int64_t n0, int64_t n1, int64_t n2, float thresh,
+                       const float* xi0, const float* xi1, const float* xi2, int* flag_dev,
+                       hipStream_t stream);
+int dmic_band_mirror_fill(float* vals, int64_t n, hipStream_t stream);
+int dmic_band_mirror_fold(const float* g, int64_t n, float* g_half, hipStream_t stream);
+
+/*
  * out[i0][i1][i2] = exp(-xi^T Sigma xi / 2), xi = (xi0[i0], xi1[i1], xi2[i2]);
  * sigma is a HOST array of 9 floats (row-major 3x3).  Replaces
  * generate_gaussian_psf (deltamic/psf.py:15-20).

@@ -200,3 +200,36 @@ def test_precision_stress_cfg3_needles(device):
     for fe, ge in results.values():
         assert fe <= FWD_TOL
         assert ge <= BWD_TOL
+
+
+def test_hermitian_halving_matches_full_band(device):
+    """Half-band evaluation + mirror equals the full-band evaluation."""
+    from deltamic_amd import hip_class
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    w = syn.config1()
+    bs, axes, otf, band = _band(w, device)
+    assert band.symmetric
+    xi = band.xi_list.cpu().numpy()
+    n = len(xi)
+    assert np.array_equal(xi[::-1], -xi)  # entry n-1-j is exactly -xi_j
+    V = torch.tensor(_shifted(w, bs), device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.tensor(w.coeff, device=device)
+    full = forward_fourier3d_cpp(V, Fc, C, band.xi_list)
+    half = hip_class.band_forward(V, Fc, C, band)
+    h = half.cpu().numpy()
+    assert np.array_equal(h[::-1][: n // 2], np.conj(h[: n // 2]))
+    assert O.rel_l2(h, full.cpu().numpy()) <= 1e-6
+    g = torch.tensor(_grad(n, 77), device=device)
+    gfull = backward_fourier3d_cpp(V, Fc, C, g, full, band.xi_list)
+    ghalf = hip_class.band_backward(V, Fc, C, g, half, band)
+    assert O.rel_l2(ghalf.cpu().numpy(), gfull.cpu().numpy()) <= 1e-6
+
+
+def test_asymmetric_band_is_not_halved(device):
+    from deltamic_amd.hip_class import compute_band
+    w = syn.config1()
+    bs, axes, otf, band = _band(w, device)
+    skew = otf * (1.0 + 0.9 * torch.tanh(axes[0] / 50.0))[:, None, None]
+    b2 = compute_band(skew.contiguous(), w.thresh, *axes)
+    assert not b2.symmetric
