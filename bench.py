@@ -71,6 +71,11 @@ def main() -> int:
                     help="band frequencies in the CPU-baseline sample (0 = skip)")
     ap.add_argument("--no-halving", action="store_true",
                     help="evaluate every band frequency instead of mirroring half of the band")
+    ap.add_argument("--full-band-steps", type=int, default=3,
+                    help="also time this many steps evaluating every band frequency")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB three times (reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--module-steps", type=int, default=3,
                     help="extra end-to-end module steps (forward, render, backward) to time")
     args = ap.parse_args()
@@ -112,9 +117,22 @@ def main() -> int:
     g_half = torch.empty((ne,), dtype=torch.complex64, device=dev)
     wsf = _lib.workspace(L.dmic_meshft_forward_workspace(nv, nf, ne), dev)
     wsb = _lib.workspace(L.dmic_meshft_backward_workspace(nv, nf, ne), dev)
+    wsf_full = _lib.workspace(L.dmic_meshft_forward_workspace(nv, nf, n), dev) if halve else wsf
+    wsb_full = _lib.workspace(L.dmic_meshft_backward_workspace(nv, nf, n), dev) if halve else wsb
     st = torch.cuda.current_stream(dev).cuda_stream
     log(f"setup {time.perf_counter() - t0:.1f} s: nv={nv} nf={nf} band={n} "
         f"symmetric={band.symmetric} evaluated={ne}")
+
+    def step_full():
+        # every band frequency evaluated (no mirroring): reported beside the headline
+        rc = L.dmic_meshft_forward(V.data_ptr(), nv, Fc.data_ptr(), 1, nf, C.data_ptr(),
+                                   band.xi_list.data_ptr(), n, out.data_ptr(), wsf_full.data_ptr(),
+                                   wsf_full.numel(), st)
+        _lib.check(rc, "forward")
+        rc = L.dmic_meshft_backward(V.data_ptr(), nv, Fc.data_ptr(), 1, nf, C.data_ptr(),
+                                    grad.data_ptr(), out.data_ptr(), band.xi_list.data_ptr(), n,
+                                    gv.data_ptr(), wsb_full.data_ptr(), wsb_full.numel(), st)
+        _lib.check(rc, "backward")
 
     def step():
         rc = L.dmic_meshft_forward(V.data_ptr(), nv, Fc.data_ptr(), 1, nf, C.data_ptr(),
@@ -135,6 +153,17 @@ def main() -> int:
         step()
         torch.cuda.synchronize()
         log(f"warmup {i + 1}/{args.warmup}")
+    if args.copy_calibration > 0:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            # an elementwise kernel (not the DMA engine a plain copy_ may use):
+            # reads nbytes, writes nbytes
+            torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"copy calibration: 3 x elementwise kernel reading {nbytes} and writing {nbytes} bytes")
     L.dmic_profile_enable(1)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -164,6 +193,22 @@ def main() -> int:
     ach_b = FLOPS_BWD * kpairs / (kb * 1e-3) / 1e12
     dom = ("meshft_bwd_kernel", ach_b, kb) if kb >= kf else ("meshft_fwd_kernel", ach_f, kf)
 
+    full_band = None
+    if halve and args.full_band_steps > 0:
+        step_full()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.full_band_steps):
+            step_full()
+        b.record()
+        torch.cuda.synchronize()
+        fms = a.elapsed_time(b) / args.full_band_steps
+        full_band = {"value": round(pairs / (fms * 1e-3) / 1e9, 3), "ms_per_step": round(fms, 4),
+                     "steps": args.full_band_steps}
+        log(f"full-band evaluation (no mirroring): {fms:.3f} ms/step, {full_band['value']} {UNIT}")
+
     # end-to-end module step: forward, render (hipFFT), normalize, loss, backward
     module_ms = None
     if args.module_steps > 0:
@@ -192,6 +237,14 @@ def main() -> int:
         log("cpu baseline ...")
         cpu = cpu_baseline(w, band.xi_list.cpu().numpy(), Vs.numpy(), args.cpu_freqs)
         log(f"cpu baseline {cpu['value']:.4f} {UNIT} on {cpu['cores']} threads")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "meshft_cfg2_pmc_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tr = json.load(fh)
+        if tr.get("band_evaluated_per_kernel") == ne and dom[0] in tr.get("bytes_per_launch", {}):
+            traffic = tr["bytes_per_launch"][dom[0]]
 
     line = {
         "metric": METRIC,
@@ -224,7 +277,7 @@ def main() -> int:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(dom[1] / FP32_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "flops_per_pair": FLOPS_BWD if dom[0] == "meshft_bwd_kernel" else FLOPS_FWD,
             "pairs_per_launch": kpairs,
             "kernel_ms": round(dom[2], 4),
@@ -236,6 +289,7 @@ def main() -> int:
             "bwd_tflops": round(ach_b, 3),
             "module_step_ms": None if module_ms is None else round(module_ms, 3),
         },
+        "full_band_evaluation": full_band,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

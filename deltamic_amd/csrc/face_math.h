@@ -73,26 +73,38 @@ __device__ __forceinline__ void sincos_rev(float u, float& s, float& c) {
   c = ((q + 1) & 2) ? -cc : cc;
 }
 
-// exp(-i x) for an FP64 phase: reduce to revolutions in FP64, then FP32.
-__device__ __forceinline__ cf expmi_phase(double x) {
+constexpr float kInv2PiF = 0.159154943091895335768883763372514362f;
+
+// Revolutions of an FP64 phase, reduced to [-1/2, 1/2] in FP64 and rounded to FP32.
+__device__ __forceinline__ float phase_rev(double x) {
   double r = x * kInv2Pi;
   r -= __builtin_rint(r);
-  float s, c;
-  sincos_rev((float)r, s, c);
-  return {c, -s};
+  return (float)r;
 }
 
-// phi(y) = (exp(-iy) - 1)/y.  Also returns s = sin(y/2), c = cos(y/2) and
-// k = sin(y/2)/(y/2) for phi'(y).
-__device__ __forceinline__ cf phi_of(float y, float& s, float& c, float& k) {
+// exp(-2 pi i u) for u in revolutions (|u| < 2 here) with the hardware
+// v_sin_f32 / v_cos_f32, which take revolutions and are accurate to ~1.2e-7
+// absolute: enough for the common phase factor of a face, whose error only
+// rotates that face's contribution.  Where relative accuracy near zero
+// matters (phi) the polynomial sincos_rev is used instead.
+__device__ __forceinline__ cf expmi_rev(float u) {
+  return {__builtin_amdgcn_cosf(u), -__builtin_amdgcn_sinf(u)};
+}
+
+// phi(y) = (exp(-iy) - 1)/y.  Also returns s = sin(y/2), c = cos(y/2),
+// k = sin(y/2)/(y/2) and ry = 1/y (0 for y ~ 0) for phi'(y).
+__device__ __forceinline__ cf phi_of(float y, float& s, float& c, float& k, float& ry) {
   sincos_rev(y * kInv4PiF, s, c);
   const float h = 0.5f * y;
-  k = (__builtin_fabsf(h) > 1e-30f) ? s * rcp(h) : 1.0f;
+  const bool nz = __builtin_fabsf(h) > 1e-30f;
+  const float rh = nz ? rcp(h) : 0.0f;
+  k = nz ? s * rh : 1.0f;
+  ry = 0.5f * rh;
   return {-k * s, -k * c};
 }
 
 // phi'(y): series for |y| < kPhiSeries, closed form otherwise.
-__device__ __forceinline__ cf dphi_of(float y, float s, float c, float k) {
+__device__ __forceinline__ cf dphi_of(float y, float s, float c, float k, float ry) {
   const float y2 = y * y;
   // series: [-1/2 + y^2/8 - y^4/144 + y^6/5760] + i y [1/3 - y^2/30 + y^4/840 - y^6/45360]
   float sr = __builtin_fmaf(y2, 1.7361111111e-4f, -6.9444444444e-3f);
@@ -104,7 +116,6 @@ __device__ __forceinline__ cf dphi_of(float y, float s, float c, float k) {
   si *= y;
   // closed form: [(-sin y + k s) + i(-cos y + k c)] / y,
   // sin y = 2 s c, cos y = 1 - 2 s^2
-  const float ry = (__builtin_fabsf(y) > 1e-30f) ? rcp(y) : 0.0f;
   const float dr = (__builtin_fmaf(-2.0f * s, c, k * s)) * ry;
   const float di = (__builtin_fmaf(2.0f * s, s, -1.0f) + k * c) * ry;
   const bool ser = __builtin_fabsf(y) < kPhiSeries;
@@ -159,9 +170,9 @@ __device__ __forceinline__ cf face_H(const Sorted& so) {
   if (so.D < kTaylorD) {
     H = taylor_H(so.P, so.Q);
   } else {
-    float sP, cP, kP, sQ, cQ, kQ;
-    const cf fP = phi_of(so.P, sP, cP, kP);
-    const cf fQ = phi_of(so.Q, sQ, cQ, kQ);
+    float sP, cP, kP, rP, sQ, cQ, kQ, rQ;
+    const cf fP = phi_of(so.P, sP, cP, kP, rP);
+    const cf fQ = phi_of(so.Q, sQ, cQ, kQ, rQ);
     const float rD = rcp(so.D);
     H = {(fP.re - fQ.re) * rD, (fP.im - fQ.im) * rD};
   }
@@ -175,13 +186,13 @@ __device__ __forceinline__ void face_H_grad(const Sorted& so, cf& H, cf& dP, cf&
     dP = taylor_dH(so.P, so.Q);
     dQ = taylor_dH(so.Q, so.P);
   } else {
-    float sP, cP, kP, sQ, cQ, kQ;
-    const cf fP = phi_of(so.P, sP, cP, kP);
-    const cf fQ = phi_of(so.Q, sQ, cQ, kQ);
+    float sP, cP, kP, rP, sQ, cQ, kQ, rQ;
+    const cf fP = phi_of(so.P, sP, cP, kP, rP);
+    const cf fQ = phi_of(so.Q, sQ, cQ, kQ, rQ);
     const float rD = rcp(so.D);
     H = {(fP.re - fQ.re) * rD, (fP.im - fQ.im) * rD};
-    const cf pP = dphi_of(so.P, sP, cP, kP);
-    const cf pQ = dphi_of(so.Q, sQ, cQ, kQ);
+    const cf pP = dphi_of(so.P, sP, cP, kP, rP);
+    const cf pQ = dphi_of(so.Q, sQ, cQ, kQ, rQ);
     dP = {(pP.re + H.re) * rD, (pP.im + H.im) * rD};
     dQ = {-(pQ.re + H.re) * rD, -(pQ.im + H.im) * rD};
   }

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
         const float p = __builtin_fmaf(r.eb[2], z, __builtin_fmaf(r.eb[1], y, r.eb[0] * x));
         const float q = __builtin_fmaf(r.ec[2], z, __builtin_fmaf(r.ec[1], y, r.ec[0] * x));
         const Sorted so = sort3(p, q);
-        const cf em = expmi_phase(xa + (double)so.mid);
+        const cf em = expmi_rev(__builtin_fmaf(so.mid, kInv2PiF, phase_rev(xa)));
         const cf H = face_H(so);
         const cf F = cmul(em, H);
         sr = __builtin_fmaf(r.w, F.re, sr);
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
         const float p = __builtin_fmaf(b2, fr.x[2], __builtin_fmaf(b1, fr.x[1], b0 * fr.x[0]));
         const float q = __builtin_fmaf(c2, fr.x[2], __builtin_fmaf(c1, fr.x[1], c0 * fr.x[0]));
         const Sorted so = sort3(p, q);
-        const cf em = expmi_phase(xa + (double)so.mid);
+        const cf em = expmi_rev(__builtin_fmaf(so.mid, kInv2PiF, phase_rev(xa)));
         cf H, dP, dQ;
         face_H_grad(so, H, dP, dQ);
         // Contract with conj(g) first: z = conj(g) e_mid, then

@@ -186,39 +186,42 @@ def trajectory_samples(problem: FitProblem, feats: Features, steps: int, every: 
 
 # ------------------------------------------------------------------ learned fit
 def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: int = 1000,
-                lr: float = 1e-3, stop_loss: float | None = None,
-                scales=(1.0, 0.5, 0.25)) -> FitResult:
-    """Accept/reject fit: at each iteration the network proposes V + s*d for
-    the step scales ``scales`` (largest first); the first proposal whose loss
-    is below the current loss is accepted, otherwise a plain Adam step is
-    taken with the gradient already computed.  ``losses[k]`` is the loss of
-    the iterate after iteration k."""
-    V = problem.init_verts.clone()
-    Vp = V.clone().requires_grad_(True)
+                lr: float = 1e-3, stop_loss: float | None = None, scale0: float = 1.0,
+                grow: float = 1.5, shrink: float = 0.5, scale_max: float = 4.0) -> FitResult:
+    """Accept/reject fit.  Each iteration the network proposes V + s*d, with a
+    trust scale s that grows after an accepted proposal and shrinks after a
+    rejected one.  The candidate's loss is computed with its autograd graph:
+    if it is below the current loss it is accepted and its gradient comes from
+    that same graph (one forward + one adjoint, the cost of an Adam step);
+    otherwise the candidate is discarded and a plain Adam step is taken with
+    the current gradient.  ``losses[k]`` is the loss after iteration k."""
+    Vp = problem.init_verts.clone().requires_grad_(True)
     opt = torch.optim.Adam([Vp], lr=lr)
     losses, times = [], []
     accepted = proposals = 0
+    s = scale0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     L, g = problem.loss_and_grad(Vp)
     for k in range(steps):
         x = feats(Vp.detach(), g)
         d = net.propose(x)
-        took = False
-        for s in scales:
-            proposals += 1
+        proposals += 1
+        cand = (Vp.detach() + s * d).requires_grad_(True)
+        Lc = problem.loss(cand)
+        if float(Lc) < float(L):
+            (gc,) = torch.autograd.grad(Lc, cand)
             with torch.no_grad():
-                Lc = problem.loss(Vp.detach() + s * d)
-            if float(Lc) < float(L):
-                with torch.no_grad():
-                    Vp.add_(s * d)
-                accepted += 1
-                took = True
-                break
-        if not took:
+                Vp.copy_(cand.detach())
+            L, g = Lc.detach(), gc
+            accepted += 1
+            s = min(s * grow, scale_max)
+        else:
+            del Lc, cand
             Vp.grad = g
             opt.step()
-        L, g = problem.loss_and_grad(Vp)
+            L, g = problem.loss_and_grad(Vp)
+            s = max(s * shrink, 1.0 / 64)
         losses.append(float(L))
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)

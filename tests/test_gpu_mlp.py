@@ -1,7 +1,10 @@
 """GPU numerics of the proposal network kernels (csrc/mlp.hip) against a
-plain PyTorch FP32 reference of the same ops (inputs rounded to bf16 as the
-matrix cores see them).  Tolerance: 1e-2 relative (bf16 has 8 bits of
-mantissa; products accumulate in FP32)."""
+plain PyTorch FP32 reference of the same ops computed on the CPU (inputs
+rounded to bf16 as the matrix cores see them).  Tolerances: 1e-4 relative
+for single GEMMs (only the FP32 summation order differs), 1e-2 / 2e-2 for the
+whole network and its gradients (activations are rounded to bf16 between
+layers on the GPU path; the reference rounds them the same way, so the
+remaining difference is FP32 summation order amplified through ReLU)."""
 import numpy as np
 import pytest
 import torch
@@ -34,13 +37,14 @@ def _gemm(A, B, a_bf16=False, b_bf16=False, transA=False, transB=False, bias=Non
                                    (128, 128, 3000)])
 def test_gemm_mfma_layout_and_values(device, M, N, K):
     g = torch.Generator(device="cpu").manual_seed(M * 7 + N)
-    A = torch.randn(M, K, generator=g).to(device)
-    B = torch.randn(K, N, generator=g).to(device)
-    ref = bf16r(A) @ bf16r(B)
-    got = _gemm(A, B)
+    A = torch.randn(M, K, generator=g)
+    B = torch.randn(K, N, generator=g)
+    ref = bf16r(A) @ bf16r(B)  # CPU reference
+    A, B = A.to(device), B.to(device)
+    got = _gemm(A, B).cpu()
     assert torch.allclose(got, ref, rtol=1e-4, atol=1e-4 * K ** 0.5)
     # transposed operands through strides
-    got_t = _gemm(A.t().contiguous(), B.t().contiguous(), transA=True, transB=True)
+    got_t = _gemm(A.t().contiguous(), B.t().contiguous(), transA=True, transB=True).cpu()
     assert torch.allclose(got_t, ref, rtol=1e-4, atol=1e-4 * K ** 0.5)
 
 
@@ -60,9 +64,10 @@ def test_gemm_bf16_storage_and_epilogues(device):
 
 
 def _torch_mlp(net, x):
-    """FP32 PyTorch reference with bf16-rounded weights and layer inputs."""
-    w = bf16r(net.w)
-    b = net.b
+    """FP32 PyTorch reference on the CPU with bf16-rounded weights and layer inputs."""
+    w = bf16r(net.w.cpu())
+    b = net.b.cpu()
+    x = x.cpu()
     sizes = [(12, 128), (128, 128), (128, 128), (128, 3)]
     Ws, off = [], 0
     for i, o in sizes:
@@ -85,9 +90,11 @@ def test_mlp_forward_backward_vs_torch(device):
     x = torch.randn(1000, 12, generator=g).to(device)
     y, acts = net.forward(x, keep_acts=True)
     ref, Ws, bs = _torch_mlp(net, x)
+    y = y.cpu()
     assert torch.allclose(y, ref, rtol=1e-2, atol=1e-2 * ref.abs().max().item())
     dy = torch.randn(1000, 3, generator=g).to(device)
     gw, gb = net.backward(x, acts, dy)
+    gw, gb, x, dy = gw.cpu(), gb.cpu(), x.cpu(), dy.cpu()
     # autograd of the reference wrt the (bf16-rounded) weights and biases
     Wr = [W.clone().requires_grad_(True) for W in Ws]
     br = [bb.clone().requires_grad_(True) for bb in bs]
