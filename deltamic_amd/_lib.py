@@ -11,7 +11,9 @@ import os
 
 import torch
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdeltamic_hip.so")
+# DELTAMIC_AMD_LIB selects another build of the same C-ABI (e.g. an A/B variant)
+LIB_PATH = os.environ.get("DELTAMIC_AMD_LIB") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "libdeltamic_hip.so")
 
 _i64 = ctypes.c_int64
 _int = ctypes.c_int

@@ -23,9 +23,9 @@
 //     dH/dP = (phi'(P) + H)/D,  dH/dQ = -(phi'(Q) + H)/D,
 // with phi'(y) from its series for |y| < 0.25 and in closed form otherwise.
 //
-// The phase x_a is computed in FP64: xi and v are FP32 values, so each
-// product is exact in FP64 and the revolution count is reduced exactly
-// before it is rounded to FP32.  Phase differences p = xi.(v_b - v_a) and
+// The phase of vertex a is computed in revolutions in FP64 from xi (FP32,
+// exact in FP64) and a / (2 pi) (FP64, relative error 1e-16), and reduced
+// exactly before it is rounded to FP32.  Phase differences p = xi.(v_b - v_a) and
 // q = xi.(v_c - v_a) come from FP32 edge vectors.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -75,11 +75,9 @@ __device__ __forceinline__ void sincos_rev(float u, float& s, float& c) {
 
 constexpr float kInv2PiF = 0.159154943091895335768883763372514362f;
 
-// Revolutions of an FP64 phase, reduced to [-1/2, 1/2] in FP64 and rounded to FP32.
-__device__ __forceinline__ float phase_rev(double x) {
-  double r = x * kInv2Pi;
-  r -= __builtin_rint(r);
-  return (float)r;
+// A phase given in revolutions (FP64), reduced to [-1/2, 1/2] and rounded to FP32.
+__device__ __forceinline__ float reduce_rev(double r) {
+  return (float)(r - __builtin_rint(r));
 }
 
 // exp(-2 pi i u) for u in revolutions (|u| < 2 here) with the hardware

@@ -38,7 +38,7 @@ void set_error(const char* fmt, ...) {
 
 // Packed per-face record of the forward tiles (64 bytes, 4 x 16B LDS reads).
 struct __align__(16) FaceRec {
-  double a[3];   // vertex a, FP64 copy of the FP32 input
+  double a[3];   // vertex a / (2 pi) in FP64: xi . a[] is the phase of a in revolutions
   float eb[3];   // v_b - v_a (FP32, correctly rounded)
   float ec[3];   // v_c - v_a
   float w;       // 2 c_t A_t / sum c A
@@ -55,9 +55,19 @@ struct __align__(16) FreqRec {
 };
 static_assert(sizeof(FreqRec) == 48, "FreqRec layout");
 
+#ifndef DMIC_BWD_ACC32
+#define DMIC_BWD_ACC32 0
+#endif
+
 constexpr int kTile = 256;        // faces (forward) / frequencies (adjoint) per LDS tile
 constexpr int kFold = 64;         // FP32 partial sums fold into FP64 every kFold pairs
-constexpr int kTargetBlocks = 2048;  // >= 8 workgroups per CU on 256 CUs
+#ifndef DMIC_TARGET_BLOCKS
+#define DMIC_TARGET_BLOCKS 2048
+#endif
+// Workgroups a launch is split into (frequency blocks x face splits, face
+// blocks x frequency chunks): several times the resident capacity, so the
+// last partial round of workgroups is a small part of the kernel.
+constexpr int kTargetBlocks = DMIC_TARGET_BLOCKS;
 
 // -------------------------------------------------------------------- prep
 
@@ -124,7 +134,7 @@ __global__ __launch_bounds__(256) void face_pack_kernel(
   const double S = S_in[0];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    r.a[d] = (double)P[0][d];
+    r.a[d] = (double)P[0][d] * kInv2Pi;
     r.eb[d] = P[1][d] - P[0][d];
     r.ec[d] = P[2][d] - P[0][d];
   }
@@ -162,11 +172,11 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
       float sr = 0.0f, si = 0.0f;
       for (int k = k0; k < k1; ++k) {
         const FaceRec& r = tile[k];
-        const double xa = fma(r.a[2], Z, fma(r.a[1], Y, r.a[0] * X));
+        const double rev_a = fma(r.a[2], Z, fma(r.a[1], Y, r.a[0] * X));
         const float p = __builtin_fmaf(r.eb[2], z, __builtin_fmaf(r.eb[1], y, r.eb[0] * x));
         const float q = __builtin_fmaf(r.ec[2], z, __builtin_fmaf(r.ec[1], y, r.ec[0] * x));
         const Sorted so = sort3(p, q);
-        const cf em = expmi_rev(__builtin_fmaf(so.mid, kInv2PiF, phase_rev(xa)));
+        const cf em = expmi_rev(__builtin_fmaf(so.mid, kInv2PiF, reduce_rev(rev_a)));
         const cf H = face_H(so);
         const cf F = cmul(em, H);
         sr = __builtin_fmaf(r.w, F.re, sr);
@@ -217,7 +227,11 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
   }
   const int64_t m0 = (int64_t)blockIdx.y * freqs_per_chunk;
   const int64_t m1 = (m0 + freqs_per_chunk < n) ? m0 + freqs_per_chunk : n;
+#if DMIC_BWD_ACC32
+  float acc[10];  // second-level FP32 sums (fewer VGPRs, one more wave per SIMD)
+#else
   double acc[10];
+#endif
 #pragma unroll
   for (int i = 0; i < 10; ++i) acc[i] = 0.0;
   for (int64_t base = m0; base < m1; base += kTile) {
@@ -241,11 +255,11 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
       for (int i = 0; i < 10; ++i) a[i] = 0.0f;
       for (int k = k0; k < k1; ++k) {
         const FreqRec& fr = tile[k];
-        const double xa = fma(A2, fr.X[2], fma(A1, fr.X[1], A0 * fr.X[0]));
+        const double rev_a = fma(A2, fr.X[2], fma(A1, fr.X[1], A0 * fr.X[0]));
         const float p = __builtin_fmaf(b2, fr.x[2], __builtin_fmaf(b1, fr.x[1], b0 * fr.x[0]));
         const float q = __builtin_fmaf(c2, fr.x[2], __builtin_fmaf(c1, fr.x[1], c0 * fr.x[0]));
         const Sorted so = sort3(p, q);
-        const cf em = expmi_rev(__builtin_fmaf(so.mid, kInv2PiF, phase_rev(xa)));
+        const cf em = expmi_rev(__builtin_fmaf(so.mid, kInv2PiF, reduce_rev(rev_a)));
         cf H, dP, dQ;
         face_H_grad(so, H, dP, dQ);
         // Contract with conj(g) first: z = conj(g) e_mid, then
@@ -271,7 +285,7 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
         a[9] += rh;
       }
 #pragma unroll
-      for (int i = 0; i < 10; ++i) acc[i] += (double)a[i];
+      for (int i = 0; i < 10; ++i) acc[i] += a[i];
     }
   }
   if (active) {

@@ -233,3 +233,46 @@ def test_asymmetric_band_is_not_halved(device):
     skew = otf * (1.0 + 0.9 * torch.tanh(axes[0] / 50.0))[:, None, None]
     b2 = compute_band(skew.contiguous(), w.thresh, *axes)
     assert not b2.symmetric
+
+
+def test_full_size_cfg2_properties(device):
+    """Config 2 at full size (all 1.08M band frequencies, 20,480 faces), checked
+    through properties that need no oracle: Hermitian symmetry of the
+    unmirrored forward, the translation phase exp(-i xi.t), invariance under
+    uniform Faces_coeff scaling, linearity of the adjoint in the upstream
+    gradient, and bitwise determinism."""
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    w = syn.config2()
+    bs, axes, otf, band = _band(w, device)
+    n = band.idx.numel()
+    V = torch.tensor(_shifted(w, bs), device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.tensor(w.coeff, device=device)
+    X = band.xi_list
+    f = forward_fourier3d_cpp(V, Fc, C, X)
+    fn = f.cpu().numpy().astype(np.complex128)
+    scale = np.abs(fn).max()
+    herm = np.abs(fn[::-1] - np.conj(fn)).max() / scale
+    # a translation on a 2^-12 grid: exact in FP32 for every vertex whose sum
+    # stays in its binade (all but a handful)
+    tt = np.array([50.0, -89.0, 65.0]) / 4096.0
+    t = torch.tensor(tt, dtype=torch.float32, device=device)
+    ft = forward_fourier3d_cpp((V + t).contiguous(), Fc, C, X).cpu().numpy().astype(np.complex128)
+    phase = np.exp(-1j * (X.cpu().numpy().astype(np.float64) @ tt))
+    trans = O.rel_l2(ft, phase * fn)
+    fs = forward_fourier3d_cpp(V, Fc, (2.5 * C).contiguous(), X)
+    g1 = torch.tensor(_grad(n, 101), device=device)
+    g2 = torch.tensor(_grad(n, 102), device=device)
+    a = backward_fourier3d_cpp(V, Fc, C, g1, f, X)
+    b = backward_fourier3d_cpp(V, Fc, C, g2, f, X)
+    ab = backward_fourier3d_cpp(V, Fc, C, (g1 + g2).contiguous(), f, X)
+    lin = O.rel_l2((a + b).cpu().numpy(), ab.cpu().numpy())
+    f2 = forward_fourier3d_cpp(V, Fc, C, X)
+    a2 = backward_fourier3d_cpp(V, Fc, C, g1, f, X)
+    print(f"cfg2 full band n={n}: hermitian {herm:.2e}, translation rel-L2 {trans:.2e}, "
+          f"adjoint linearity rel-L2 {lin:.2e}")
+    assert herm <= 1e-6
+    assert trans <= 1e-5
+    assert torch.equal(fs.view(torch.float32), f.view(torch.float32))
+    assert lin <= 1e-6
+    assert torch.equal(f2.view(torch.float32), f.view(torch.float32)) and torch.equal(a2, a)

@@ -185,3 +185,25 @@ def test_uniform_laplacian_adjoint_matches_dense_autograd():
     (g2,) = torch.autograd.grad(val2, V2)
     assert abs(float(val) - float(val2)) <= 1e-12
     assert torch.allclose(g, g2, atol=1e-12)
+
+
+def test_proposal_features_on_a_sphere():
+    """Per-vertex features: centred/scaled position, outward unit normal,
+    gradient scaled to unit RMS and its 1-ring mean."""
+    from deltamic_amd.fit import UniformLaplacian
+    from deltamic_amd.proposal import Features
+    u, f = syn.icosphere(3)
+    V = torch.tensor(2.0 * u + 0.5, dtype=torch.float32)
+    lap = UniformLaplacian(f, len(u), "cpu")
+    feats = Features(f, len(u), lap, "cpu")
+    g = torch.tensor(syn.rng(3).normal(size=u.shape), dtype=torch.float32)
+    x = feats(V, 7.0 * g)
+    assert x.shape == (len(u), 12)
+    P, n, gg, gm = x[:, 0:3], x[:, 3:6], x[:, 6:9], x[:, 9:12]
+    assert torch.allclose(P.pow(2).sum(1).mean(), torch.tensor(1.0), atol=1e-5)
+    assert torch.allclose(n.norm(dim=1), torch.ones(len(u)), atol=1e-5)
+    assert (torch.sum(n * torch.tensor(u, dtype=torch.float32), 1) > 0.99).all()  # outward
+    assert torch.allclose(gg.pow(2).sum(1).mean(), torch.tensor(1.0), atol=1e-5)
+    i = 17
+    nb = lap.nbr[i][lap.mask[i] > 0]
+    assert torch.allclose(gm[i], gg[nb].mean(0), atol=1e-6)
