@@ -55,18 +55,16 @@ struct __align__(16) FreqRec {
 };
 static_assert(sizeof(FreqRec) == 48, "FreqRec layout");
 
-#ifndef DMIC_BWD_ACC32
-#define DMIC_BWD_ACC32 0
-#endif
-
 constexpr int kTile = 256;        // faces (forward) / frequencies (adjoint) per LDS tile
 constexpr int kFold = 64;         // FP32 partial sums fold into FP64 every kFold pairs
 #ifndef DMIC_TARGET_BLOCKS
-#define DMIC_TARGET_BLOCKS 2048
+#define DMIC_TARGET_BLOCKS 16384
 #endif
 // Workgroups a launch is split into (frequency blocks x face splits, face
-// blocks x frequency chunks): several times the resident capacity, so the
-// last partial round of workgroups is a small part of the kernel.
+// blocks x frequency chunks).  The resident capacity is 256 CUs x 2-3
+// workgroups; 16384 makes the last partial round of workgroups a few percent
+// of the kernel (on the 1.08M-frequency band of a 20480-face sphere: 2048 -> 64.7
+// ms per forward+adjoint, 8192 -> 53.8 ms, 16384 -> 52.6 ms on MI355X).
 constexpr int kTargetBlocks = DMIC_TARGET_BLOCKS;
 
 // -------------------------------------------------------------------- prep
@@ -227,11 +225,7 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
   }
   const int64_t m0 = (int64_t)blockIdx.y * freqs_per_chunk;
   const int64_t m1 = (m0 + freqs_per_chunk < n) ? m0 + freqs_per_chunk : n;
-#if DMIC_BWD_ACC32
-  float acc[10];  // second-level FP32 sums (fewer VGPRs, one more wave per SIMD)
-#else
   double acc[10];
-#endif
 #pragma unroll
   for (int i = 0; i < 10; ++i) acc[i] = 0.0;
   for (int64_t base = m0; base < m1; base += kTile) {

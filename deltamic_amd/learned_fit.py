@@ -36,18 +36,24 @@ def log(*a):
 
 
 def collect(args, device):
+    """Features with gradients scaled by nv (grad_scale = 1) and their targets."""
     xs, ts = [], []
     for k in range(args.train_fits):
         t0 = time.perf_counter()
         w = syn.config4(k, subdivisions=args.subdiv, box=args.box)
         prob = make_problem(w, device)
-        feats = Features(w.faces, len(w.extra["init_verts"]), prob.lap, device)
+        feats = Features(w.faces, len(w.extra["init_verts"]), prob.lap, device, grad_scale=1.0)
         d = trajectory_samples(prob, feats, args.train_steps, args.every, args.horizon)
         xs.append(d.x)
         ts.append(d.target)
         log(f"trajectory {k + 1}/{args.train_fits}: {d.x.shape[0]} rows "
             f"({time.perf_counter() - t0:.1f} s)")
-    return torch.cat(xs), torch.cat(ts)
+    x = torch.cat(xs)
+    # calibrate the gradient feature scale to unit RMS over the training set
+    rms = float(x[:, 6:9].pow(2).sum(1).mean().sqrt())
+    gs = 1.0 / max(rms, 1e-30)
+    x[:, 6:12] *= gs
+    return x, torch.cat(ts), gs
 
 
 def train(net, x, t, epochs, batch, lr, seed):
@@ -70,7 +76,8 @@ def train(net, x, t, epochs, batch, lr, seed):
 def evaluate(net, args, device, subdiv, k):
     w = syn.config4(10000 + k, subdivisions=subdiv, box=args.box)
     prob = make_problem(w, device)
-    feats = Features(w.faces, len(w.extra["init_verts"]), prob.lap, device)
+    feats = Features(w.faces, len(w.extra["init_verts"]), prob.lap, device,
+                     grad_scale=net.grad_scale)
     # warm the band cache and kernels outside the timed fits
     prob.loss_and_grad(prob.init_verts)
     torch.cuda.synchronize()
@@ -103,7 +110,7 @@ def main(argv=None):
     ap.add_argument("--every", type=int, default=10)
     ap.add_argument("--horizon", type=int, default=100)
     ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval-shapes", type=int, default=4)
     ap.add_argument("--eval-shapes-large", type=int, default=1)
@@ -121,7 +128,9 @@ def main(argv=None):
         net.load_state_dict(torch.load(args.load, weights_only=True))
     else:
         t0 = time.perf_counter()
-        x, t = collect(args, device)
+        x, t, gs = collect(args, device)
+        net.grad_scale = gs
+        summary["grad_scale"] = gs
         summary["train_rows"] = int(x.shape[0])
         summary["collect_s"] = time.perf_counter() - t0
         t1 = time.perf_counter()
@@ -129,16 +138,16 @@ def main(argv=None):
         summary["train_s"] = time.perf_counter() - t1
         torch.save(net.state_dict(), os.path.join(args.out, "proposal.pt"))
     evals = []
-    for k in range(args.eval_shapes):
-        evals.append(evaluate(net, args, device, args.subdiv, k))
-    for k in range(args.eval_shapes_large):
-        evals.append(evaluate(net, args, device, args.subdiv + 1, 100 + k))
-    summary["eval"] = evals
-    sp = [e["speedup"] for e in evals if e["speedup"] is not None]
-    summary["median_speedup"] = float(np.median(sp)) if sp else None
-    summary["reached"] = f"{len(sp)}/{len(evals)}"
-    with open(os.path.join(args.out, "summary.json"), "w") as f:
-        json.dump(summary, f, indent=1)
+    jobs = [(args.subdiv, k) for k in range(args.eval_shapes)]
+    jobs += [(args.subdiv + 1, 100 + k) for k in range(args.eval_shapes_large)]
+    for subdiv, k in jobs:
+        evals.append(evaluate(net, args, device, subdiv, k))
+        summary["eval"] = evals
+        sp = [e["speedup"] for e in evals if e["speedup"] is not None]
+        summary["median_speedup"] = float(np.median(sp)) if sp else None
+        summary["reached"] = f"{len(sp)}/{len(evals)}"
+        with open(os.path.join(args.out, "summary.json"), "w") as f:  # after every shape
+            json.dump(summary, f, indent=1)
     log(json.dumps({k: summary[k] for k in ("median_speedup", "reached")}))
 
 

@@ -5,7 +5,10 @@ bf16 weights, FP32 accumulation; HIP kernels in csrc/mlp.hip) maps
 per-vertex features
 
     position (centred, scaled by the mesh RMS radius), unit vertex normal,
-    loss gradient and its 1-ring mean (both scaled by the mesh RMS gradient)
+    loss gradient and its 1-ring mean (both multiplied by nv * grad_scale:
+    per-vertex gradients scale like the vertex area ~ 1/nv, and grad_scale
+    is calibrated on the training trajectories, so the features keep the
+    absolute gradient size the step length depends on)
 
 to a displacement.  The network is trained on the method's own plain-Adam
 fitting trajectories: the target for a snapshot (V_k, grad_k) is the
@@ -46,12 +49,15 @@ def vertex_face_table(faces: np.ndarray, nv: int):
 class Features:
     """Per-vertex feature builder for one mesh topology (all torch gathers: deterministic)."""
 
-    def __init__(self, faces_np: np.ndarray, nv: int, lap, device):
+    def __init__(self, faces_np: np.ndarray, nv: int, lap, device, grad_scale: float | None = None):
         tab, mask = vertex_face_table(faces_np, nv)
         self.faces = torch.tensor(np.asarray(faces_np), device=device)
         self.vf = torch.tensor(tab, device=device)
         self.vf_mask = torch.tensor(mask, device=device)
         self.lap = lap
+        self.nv = nv
+        # None: scale gradients to unit RMS over the mesh (direction only)
+        self.grad_scale = grad_scale
 
     @torch.no_grad()
     def __call__(self, V: torch.Tensor, grad: torch.Tensor) -> torch.Tensor:
@@ -62,8 +68,10 @@ class Features:
         fn = torch.cross(b - a, cc - a, dim=1)  # area-weighted face normals
         vn = (fn[self.vf] * self.vf_mask[..., None]).sum(1)
         vn = vn / vn.norm(dim=1, keepdim=True).clamp_min(1e-20)
-        gs = grad.pow(2).sum(1).mean().sqrt().clamp_min(1e-30)
-        g = grad / gs
+        if self.grad_scale is None:
+            g = grad / grad.pow(2).sum(1).mean().sqrt().clamp_min(1e-30)
+        else:
+            g = grad * (self.nv * self.grad_scale)
         lap = self.lap
         gm = (g[lap.nbr] * lap.mask[..., None]).sum(1) * lap.invdeg[:, None]
         return torch.cat([P / rad, vn, g, gm], dim=1).contiguous()
@@ -72,12 +80,14 @@ class Features:
 class ProposalMLP:
     """The proposal network: fp32 master parameters, bf16 weight copy, Adam state."""
 
-    def __init__(self, device, seed: int = 0, out_scale: float = 0.02):
+    def __init__(self, device, seed: int = 0, out_scale: float = 0.02,
+                 grad_scale: float | None = None):
         L = _lib.lib()
         self.nw = int(L.dmic_mlp_weight_count())
         self.nb = int(L.dmic_mlp_bias_count())
         self.device = torch.device(device)
         self.out_scale = out_scale
+        self.grad_scale = grad_scale  # feature scaling the network was trained with
         g = torch.Generator().manual_seed(seed)
         chunks = []
         for fan_in, fan_out in ((N_IN, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_OUT)):
@@ -100,12 +110,15 @@ class ProposalMLP:
 
     def state_dict(self):
         return {"w": self.w.cpu(), "b": self.b.cpu(), "out_scale": self.out_scale,
+                "grad_scale": -1.0 if self.grad_scale is None else float(self.grad_scale),
                 "step": self.step_count}
 
     def load_state_dict(self, sd):
         self.w = sd["w"].to(self.device).contiguous()
         self.b = sd["b"].to(self.device).contiguous()
         self.out_scale = float(sd["out_scale"])
+        gsc = float(sd.get("grad_scale", -1.0))
+        self.grad_scale = None if gsc < 0 else gsc
         self.step_count = int(sd.get("step", 0))
         self._sync_bf16()
 
@@ -185,43 +198,73 @@ def trajectory_samples(problem: FitProblem, feats: Features, steps: int, every: 
 
 
 # ------------------------------------------------------------------ learned fit
+def _adam_predicted_decrease(opt: torch.optim.Adam, p: torch.Tensor, g: torch.Tensor) -> float:
+    """First-order loss change g . delta of the Adam step opt would take with gradient g."""
+    group = opt.param_groups[0]
+    b1, b2 = group["betas"]
+    lr, eps = group["lr"], group["eps"]
+    st = opt.state.get(p, {})
+    t = int(st.get("step", 0)) + 1
+    m = st["exp_avg"] * b1 + (1 - b1) * g if "exp_avg" in st else (1 - b1) * g
+    v = st["exp_avg_sq"] * b2 + (1 - b2) * g * g if "exp_avg_sq" in st else (1 - b2) * g * g
+    step = lr * (m / (1 - b1 ** t)) / ((v / (1 - b2 ** t)).sqrt() + eps)
+    return float(-(g * step).sum())
+
+
 def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: int = 1000,
                 lr: float = 1e-3, stop_loss: float | None = None, scale0: float = 1.0,
-                grow: float = 1.5, shrink: float = 0.5, scale_max: float = 4.0) -> FitResult:
-    """Accept/reject fit.  Each iteration the network proposes V + s*d, with a
-    trust scale s that grows after an accepted proposal and shrinks after a
-    rejected one.  The candidate's loss is computed with its autograd graph:
-    if it is below the current loss it is accepted and its gradient comes from
-    that same graph (one forward + one adjoint, the cost of an Adam step);
-    otherwise the candidate is discarded and a plain Adam step is taken with
-    the current gradient.  ``losses[k]`` is the loss after iteration k."""
+                grow: float = 1.5, shrink: float = 0.5, scale_max: float = 8.0,
+                patience: int = 4, cooldown: int = 8) -> FitResult:
+    """Accept/reject fit.  Each iteration the network proposes V + s*d (trust
+    scale s grows after an accepted proposal, shrinks after a rejection).
+    The candidate's loss is computed with its autograd graph and accepted only
+    if it is lower than the current loss by more than the first-order
+    decrease of the Adam step that would otherwise be taken; its gradient then
+    comes from that same graph (one forward + one adjoint, the cost of an Adam
+    step).  Otherwise the candidate is discarded and the plain Adam step is
+    taken.  After ``patience`` rejections in a row no proposal is made for
+    ``cooldown`` iterations (plain Adam).  ``losses[k]`` is the loss after
+    iteration k."""
     Vp = problem.init_verts.clone().requires_grad_(True)
     opt = torch.optim.Adam([Vp], lr=lr)
     losses, times = [], []
     accepted = proposals = 0
     s = scale0
+    misses = 0
+    quiet = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     L, g = problem.loss_and_grad(Vp)
     for k in range(steps):
-        x = feats(Vp.detach(), g)
-        d = net.propose(x)
-        proposals += 1
-        cand = (Vp.detach() + s * d).requires_grad_(True)
-        Lc = problem.loss(cand)
-        if float(Lc) < float(L):
-            (gc,) = torch.autograd.grad(Lc, cand)
-            with torch.no_grad():
-                Vp.copy_(cand.detach())
-            L, g = Lc.detach(), gc
-            accepted += 1
-            s = min(s * grow, scale_max)
+        took = False
+        if quiet > 0:
+            quiet -= 1
         else:
-            del Lc, cand
+            x = feats(Vp.detach(), g)
+            d = net.propose(x)
+            proposals += 1
+            cand = (Vp.detach() + s * d).requires_grad_(True)
+            Lc = problem.loss(cand)
+            bar = float(L) + _adam_predicted_decrease(opt, Vp, g)
+            if float(Lc) < bar:
+                (gc,) = torch.autograd.grad(Lc, cand)
+                with torch.no_grad():
+                    Vp.copy_(cand.detach())
+                L, g = Lc.detach(), gc
+                accepted += 1
+                took = True
+                misses = 0
+                s = min(s * grow, scale_max)
+            else:
+                del Lc, cand
+                s = max(s * shrink, 1.0 / 64)
+                misses += 1
+                if misses >= patience:
+                    quiet, misses = cooldown, 0
+        if not took:
             Vp.grad = g
             opt.step()
             L, g = problem.loss_and_grad(Vp)
-            s = max(s * shrink, 1.0 / 64)
         losses.append(float(L))
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)

@@ -90,7 +90,8 @@ def test_trained_proposal_gains_over_plain_adam(device):
     w = syn.config4(10500)  # held out: training used seeds 4000 + 0..31
     from deltamic_amd.fit import make_problem
     prob = make_problem(w, device)
-    feats = Features(w.faces, prob.init_verts.shape[0], prob.lap, device)
+    feats = Features(w.faces, prob.init_verts.shape[0], prob.lap, device,
+                     grad_scale=net.grad_scale)
     prob.loss_and_grad(prob.init_verts)
     adam = fit_adam(prob, steps=200)
     target = adam.losses[-1]

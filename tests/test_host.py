@@ -39,8 +39,11 @@ def test_library_exports_every_declared_symbol():
 def test_workspace_sizes_are_sane():
     from deltamic_amd import _lib
     L = _lib.lib()
-    assert L.dmic_meshft_forward_workspace(10242, 20480, 815400) < 64 << 20
-    assert L.dmic_meshft_backward_workspace(10242, 20480, 815400) < 256 << 20
+    # partial-sum buffers are bounded by the launch split (16384 workgroups x
+    # 256 lanes x 16 B forward / 80 B adjoint), not by the band size
+    assert L.dmic_meshft_forward_workspace(10242, 20480, 815400) < 128 << 20
+    assert L.dmic_meshft_backward_workspace(10242, 20480, 815400) < 512 << 20
+    assert L.dmic_meshft_backward_workspace(50002, 100000, 10 ** 7) < 512 << 20
     assert L.dmic_meshft_forward_workspace(642, 1280, 22000) > 0
     assert L.dmic_band_workspace(200, 200, 200) > 0
     assert L.dmic_meshft_forward_workspace(0, 0, 0) > 0
