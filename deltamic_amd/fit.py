@@ -140,6 +140,7 @@ class FitResult:
     accepted: int = 0
     proposals: int = 0
     snapshots: list = field(default_factory=list)
+    accepted_at: list = field(default_factory=list)  # learned fit: iterations that took a proposal
 
 
 def fit_adam(problem: FitProblem, steps: int = 1000, lr: float = 1e-3, snapshot_every: int = 0,

@@ -229,6 +229,7 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
     opt = torch.optim.Adam([Vp], lr=lr)
     losses, times = [], []
     accepted = proposals = 0
+    accepted_at = []
     s = scale0
     misses = 0
     quiet = 0
@@ -246,12 +247,13 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
             cand = (Vp.detach() + s * d).requires_grad_(True)
             Lc = problem.loss(cand)
             bar = float(L) + _adam_predicted_decrease(opt, Vp, g)
-            if float(Lc) < bar:
+            if float(Lc.detach()) < bar:
                 (gc,) = torch.autograd.grad(Lc, cand)
                 with torch.no_grad():
                     Vp.copy_(cand.detach())
                 L, g = Lc.detach(), gc
                 accepted += 1
+                accepted_at.append(k)
                 took = True
                 misses = 0
                 s = min(s * grow, scale_max)
@@ -271,4 +273,4 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
         if stop_loss is not None and losses[-1] <= stop_loss:
             break
     return FitResult(verts=Vp.detach(), losses=losses, times=times, accepted=accepted,
-                     proposals=proposals)
+                     proposals=proposals, accepted_at=accepted_at)
