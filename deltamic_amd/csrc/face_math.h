@@ -21,12 +21,20 @@
 //     dF/dx_lo = e_m dH/dP,  dF/dx_hi = e_m dH/dQ,
 //     dF/dx_mid = -iF - dF/dx_lo - dF/dx_hi,
 //     dH/dP = (phi'(P) + H)/D,  dH/dQ = -(phi'(Q) + H)/D,
-// with phi'(y) from its series for |y| < 0.25 and in closed form otherwise.
+// with Im phi'(y) from its series for |y| < 1 and in closed form otherwise.
 //
 // The phase of vertex a is computed in revolutions in FP64 from xi (FP32,
 // exact in FP64) and a / (2 pi) (FP64, relative error 1e-16), and reduced
 // exactly before it is rounded to FP32.  Phase differences p = xi.(v_b - v_a) and
 // q = xi.(v_c - v_a) come from FP32 edge vectors.
+//
+// sin and cos are the hardware v_sin_f32 / v_cos_f32, which take revolutions
+// (measured on MI355X: 1.2e-7 absolute, 8.7e-7 relative down to 1e-9
+// revolutions).  Their input domain is |u| <= 256 revolutions.  Faces whose
+// phase gap D exceeds kWidePhase take the "wide" variant, which reduces every
+// argument to [-1/2, 1/2] revolutions first (the median-vertex phase in
+// FP64); the others (|mid| <= D, so |x_mid| < 0.5 + D/2pi and |y/2| <= D/2
+// stay in the domain) skip the reduction.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -35,8 +43,13 @@ namespace dmic {
 
 constexpr double kInv2Pi = 0.159154943091895335768883763372514362;
 constexpr float kInv4PiF = 0.0795774715459476678844418816862571810f;
+constexpr float kInv4PiLoF = 3.2103191632826267e-09f;  // 1/(4 pi) - kInv4PiF
 constexpr float kTaylorD = 0.1f;    // largest phase gap handled by the Taylor branch
-constexpr float kPhiSeries = 0.25f;  // |y| below which phi'(y) uses its series
+constexpr float kPhiSeries = 1.0f;   // |y| below which Im phi'(y) uses its series
+constexpr float kWidePhase = 1000.0f;  // phase gap above which arguments are range-reduced
+// P and Q are kept at least kTinyY away from 0 (phi moves by < 7e-9 there),
+// so one reciprocal of P Q D gives 1/P, 1/Q and 1/D.
+constexpr float kTinyY = 1.3e-8f;
 
 struct cf {
   float re, im;
@@ -49,77 +62,42 @@ __device__ __forceinline__ cf cmul(cf a, cf b) {
   return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
-// sin(2 pi u) and cos(2 pi u) for u in revolutions.  The octant reduction
-// u = n/4 + f, |f| <= 1/8, is exact in FP32.  sin(2 pi f) = f P(f^2) and
-// cos(2 pi f) = Q(f^2) are near-minimax fits on |f| <= 1/8 (relative error
-// 3.2e-9 and 5.6e-11; fitted for this file by weighted least squares on
-// Chebyshev nodes), so small arguments keep full relative accuracy.
-__device__ __forceinline__ void sincos_rev(float u, float& s, float& c) {
-  const float n = __builtin_rintf(u * 4.0f);
-  const float f = __builtin_fmaf(n, -0.25f, u);
-  const float f2 = f * f;
-  float sp = __builtin_fmaf(f2, -75.39373016f, 81.59236145f);
-  sp = __builtin_fmaf(f2, sp, -41.34166336f);
-  sp = __builtin_fmaf(f2, sp, 6.283185482f);
-  sp *= f;
-  float cp = __builtin_fmaf(f2, 59.20288849f, -85.44229889f);
-  cp = __builtin_fmaf(f2, cp, 64.93930817f);
-  cp = __builtin_fmaf(f2, cp, -19.73920822f);
-  cp = __builtin_fmaf(f2, cp, 1.0f);
-  const int q = ((int)n) & 3;
-  const float ss = (q & 1) ? cp : sp;
-  const float cc = (q & 1) ? sp : cp;
-  s = (q & 2) ? -ss : ss;
-  c = ((q + 1) & 2) ? -cc : cc;
-}
-
 constexpr float kInv2PiF = 0.159154943091895335768883763372514362f;
 
-// A phase given in revolutions (FP64), reduced to [-1/2, 1/2] and rounded to FP32.
+// A phase given in revolutions (FP64), reduced to [0, 1) (v_fract_f64) and
+// rounded to FP32 (absolute error <= 3e-8 revolutions).
 __device__ __forceinline__ float reduce_rev(double r) {
-  return (float)(r - __builtin_rint(r));
+  return (float)__builtin_amdgcn_fract(r);
 }
 
-// exp(-2 pi i u) for u in revolutions (|u| < 2 here) with the hardware
-// v_sin_f32 / v_cos_f32, which take revolutions and are accurate to ~1.2e-7
-// absolute: enough for the common phase factor of a face, whose error only
-// rotates that face's contribution.  Where relative accuracy near zero
-// matters (phi) the polynomial sincos_rev is used instead.
+// exp(-2 pi i u) for u in revolutions (|u| <= 256) with the hardware sin/cos.
+// Only the common phase factor of a face uses it, where an absolute error
+// just rotates that face's contribution.
 __device__ __forceinline__ cf expmi_rev(float u) {
   return {__builtin_amdgcn_cosf(u), -__builtin_amdgcn_sinf(u)};
 }
-
-// phi(y) = (exp(-iy) - 1)/y.  Also returns s = sin(y/2), c = cos(y/2),
-// k = sin(y/2)/(y/2) and ry = 1/y (0 for y ~ 0) for phi'(y).
-__device__ __forceinline__ cf phi_of(float y, float& s, float& c, float& k, float& ry) {
-  sincos_rev(y * kInv4PiF, s, c);
-  const float h = 0.5f * y;
-  const bool nz = __builtin_fabsf(h) > 1e-30f;
-  const float rh = nz ? rcp(h) : 0.0f;
-  k = nz ? s * rh : 1.0f;
-  ry = 0.5f * rh;
-  return {-k * s, -k * c};
+// Phase of the median vertex in revolutions, reduced: FP32 sum for ordinary
+// faces, FP64 sum and exact reduction for wide ones (|mid| up to any size).
+template <bool kWide>
+__device__ __forceinline__ float mid_rev(double rev_a, float mid) {
+  if (kWide) return reduce_rev(fma((double)mid, kInv2Pi, rev_a));
+  return __builtin_fmaf(mid, kInv2PiF, reduce_rev(rev_a));
 }
-
-// phi'(y): series for |y| < kPhiSeries, closed form otherwise.
+// phi'(y) = [(-sin y + k s) + i(-cos y + k c)] / y.
+// Real part = (k/2)(k - 2c): no cancellation for any y (k, c -> 1 as y -> 0).
+// Imaginary part = (k c + 2 s^2 - 1)/y cancels for small y; below kPhiSeries
+// it comes from its series y (1/3 - y^2/30 + y^4/840 - y^6/45360 +
+// y^8/3991680) (truncation < 2e-9 at |y| = 1).
 __device__ __forceinline__ cf dphi_of(float y, float s, float c, float k, float ry) {
+  const float re = 0.5f * k * __builtin_fmaf(-2.0f, c, k);
   const float y2 = y * y;
-  // series: [-1/2 + y^2/8 - y^4/144 + y^6/5760] + i y [1/3 - y^2/30 + y^4/840 - y^6/45360]
-  float sr = __builtin_fmaf(y2, 1.7361111111e-4f, -6.9444444444e-3f);
-  sr = __builtin_fmaf(y2, sr, 0.125f);
-  sr = __builtin_fmaf(y2, sr, -0.5f);
-  float si = __builtin_fmaf(y2, -2.2045855379e-5f, 1.1904761905e-3f);
+  float si = __builtin_fmaf(y2, 2.5052108385e-7f, -2.2045855379e-5f);
+  si = __builtin_fmaf(y2, si, 1.1904761905e-3f);
   si = __builtin_fmaf(y2, si, -3.3333333333e-2f);
-  si = __builtin_fmaf(y2, si, 3.3333333333e-1f);
-  si *= y;
-  // closed form: [(-sin y + k s) + i(-cos y + k c)] / y,
-  // sin y = 2 s c, cos y = 1 - 2 s^2
-  const float dr = (__builtin_fmaf(-2.0f * s, c, k * s)) * ry;
-  const float di = (__builtin_fmaf(2.0f * s, s, -1.0f) + k * c) * ry;
-  const bool ser = __builtin_fabsf(y) < kPhiSeries;
-  return {ser ? sr : dr, ser ? si : di};
+  si = __builtin_fmaf(y2, si, 3.3333333333e-1f) * y;
+  const float di = __builtin_fmaf(k, c, __builtin_fmaf(2.0f * s, s, -1.0f)) * ry;
+  return {re, __builtin_fabsf(y) < kPhiSeries ? si : di};
 }
-
 // Taylor branch: H(P,Q) = sum_k -(-i)^k h_{k-2}(P,Q)/k!, k = 2..8.
 __device__ __forceinline__ cf taylor_H(float P, float Q) {
   const float s1 = P + Q, t1 = P * Q;
@@ -162,38 +140,71 @@ __device__ __forceinline__ Sorted sort3(float p, float q) {
   return {mid, lo - mid, hi - mid, hi - lo};
 }
 
-// H(P, Q) for the forward.
-__device__ __forceinline__ cf face_H(const Sorted& so) {
-  cf H;
-  if (so.D < kTaylorD) {
-    H = taylor_H(so.P, so.Q);
-  } else {
-    float sP, cP, kP, rP, sQ, cQ, kQ, rQ;
-    const cf fP = phi_of(so.P, sP, cP, kP, rP);
-    const cf fQ = phi_of(so.Q, sQ, cQ, kQ, rQ);
-    const float rD = rcp(so.D);
-    H = {(fP.re - fQ.re) * rD, (fP.im - fQ.im) * rD};
+// sin(y/2), cos(y/2) with the hardware sin/cos (kWide: reduced argument).
+template <bool kWide>
+__device__ __forceinline__ void half_sincos(float y, float& s, float& c) {
+  float u = y * kInv4PiF;
+  if (kWide) {
+    // y/(4 pi) with the rounding error of the product and of the constant
+    // restored, then reduced: accurate to ~3e-8 revolutions for any y
+    const float lo = __builtin_fmaf(y, kInv4PiF, -u) + y * kInv4PiLoF;
+    u = (u - __builtin_rintf(u)) + lo;
   }
-  return H;
+  s = __builtin_amdgcn_sinf(u);
+  c = __builtin_amdgcn_cosf(u);
 }
-
-// H and its derivatives for the adjoint.
-__device__ __forceinline__ void face_H_grad(const Sorted& so, cf& H, cf& dP, cf& dQ) {
+// H(P, Q) for the forward, returned as H = m * N (the kernel folds m into
+// the face weight):
+//     H = (phi(P) - phi(Q))/D = (Q A_P - P A_Q) / (P Q D),
+//     A_y = exp(-iy) - 1 = -2 s (s + i c),  s = sin(y/2), c = cos(y/2),
+// so N = sP Q (sP + i cP) - sQ P (sQ + i cQ) and m = -2/(P Q D): one
+// reciprocal per pair.
+template <bool kWide>
+__device__ __forceinline__ cf face_H(const Sorted& so, float& m) {
+  cf N;
   if (so.D < kTaylorD) {
-    H = taylor_H(so.P, so.Q);
-    dP = taylor_dH(so.P, so.Q);
-    dQ = taylor_dH(so.Q, so.P);
+    N = taylor_H(so.P, so.Q);
+    m = 1.0f;
   } else {
-    float sP, cP, kP, rP, sQ, cQ, kQ, rQ;
-    const cf fP = phi_of(so.P, sP, cP, kP, rP);
-    const cf fQ = phi_of(so.Q, sQ, cQ, kQ, rQ);
-    const float rD = rcp(so.D);
-    H = {(fP.re - fQ.re) * rD, (fP.im - fQ.im) * rD};
-    const cf pP = dphi_of(so.P, sP, cP, kP, rP);
-    const cf pQ = dphi_of(so.Q, sQ, cQ, kQ, rQ);
-    dP = {(pP.re + H.re) * rD, (pP.im + H.im) * rD};
-    dQ = {-(pQ.re + H.re) * rD, -(pQ.im + H.im) * rD};
+    const float P = __builtin_fminf(so.P, -kTinyY), Q = __builtin_fmaxf(so.Q, kTinyY);
+    float sP, cP, sQ, cQ;
+    half_sincos<kWide>(P, sP, cP);
+    half_sincos<kWide>(Q, sQ, cQ);
+    const float u = sP * Q, v = sQ * P;
+    N = {__builtin_fmaf(u, sP, -v * sQ), __builtin_fmaf(u, cP, -v * cQ)};
+    m = -2.0f * rcp(P * Q * so.D);
+  }
+  return N;
+}
+// H and its derivatives for the adjoint, returned as (d, pP, pQ, rD) with
+//     H = d rD,  dH/dP = (pP + H) rD,  dH/dQ = -(pQ + H) rD,
+// i.e. d = phi(P) - phi(Q), pP = phi'(P), pQ = phi'(Q), rD = 1/D in the
+// closed form (phi(y) = -k (s + i c), k = sin(y/2)/(y/2) = 2 s / y), and
+// d = H, rD = 1 in the Taylor branch.  The kernel contracts with the
+// upstream gradient before scaling by rD.
+template <bool kWide>
+__device__ __forceinline__ void face_H_grad(const Sorted& so, cf& d, cf& pP, cf& pQ, float& rD) {
+  if (so.D < kTaylorD) {
+    d = taylor_H(so.P, so.Q);
+    const cf tP = taylor_dH(so.P, so.Q);
+    const cf tQ = taylor_dH(so.Q, so.P);
+    pP = {tP.re - d.re, tP.im - d.im};
+    pQ = {-tQ.re - d.re, -tQ.im - d.im};
+    rD = 1.0f;
+  } else {
+    const float P = __builtin_fminf(so.P, -kTinyY), Q = __builtin_fmaxf(so.Q, kTinyY);
+    float sP, cP, sQ, cQ;
+    half_sincos<kWide>(P, sP, cP);
+    half_sincos<kWide>(Q, sQ, cQ);
+    const float PQ = P * Q;
+    const float r = rcp(PQ * so.D);
+    const float Dr = so.D * r;
+    const float rP = Q * Dr, rQ = P * Dr;   // 1/P, 1/Q
+    rD = PQ * r;
+    const float kP = 2.0f * sP * rP, kQ = 2.0f * sQ * rQ;
+    d = {kQ * sQ - kP * sP, kQ * cQ - kP * cP};
+    pP = dphi_of(P, sP, cP, kP, rP);
+    pQ = dphi_of(Q, sQ, cQ, kQ, rQ);
   }
 }
-
 }  // namespace dmic

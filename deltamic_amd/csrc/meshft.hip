@@ -174,11 +174,15 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
         const float p = __builtin_fmaf(r.eb[2], z, __builtin_fmaf(r.eb[1], y, r.eb[0] * x));
         const float q = __builtin_fmaf(r.ec[2], z, __builtin_fmaf(r.ec[1], y, r.ec[0] * x));
         const Sorted so = sort3(p, q);
-        const cf em = expmi_rev(__builtin_fmaf(so.mid, kInv2PiF, reduce_rev(rev_a)));
-        const cf H = face_H(so);
-        const cf F = cmul(em, H);
-        sr = __builtin_fmaf(r.w, F.re, sr);
-        si = __builtin_fmaf(r.w, F.im, si);
+        cf F;   // F = e_mid N, face value = m F
+        float m;
+        if (so.D <= kWidePhase)
+          F = cmul(expmi_rev(mid_rev<false>(rev_a, so.mid)), face_H<false>(so, m));
+        else
+          F = cmul(expmi_rev(mid_rev<true>(rev_a, so.mid)), face_H<true>(so, m));
+        const float wm = r.w * m;
+        sr = __builtin_fmaf(wm, F.re, sr);
+        si = __builtin_fmaf(wm, F.im, si);
       }
       accr += (double)sr;
       acci += (double)si;
@@ -253,28 +257,35 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
         const float p = __builtin_fmaf(b2, fr.x[2], __builtin_fmaf(b1, fr.x[1], b0 * fr.x[0]));
         const float q = __builtin_fmaf(c2, fr.x[2], __builtin_fmaf(c1, fr.x[1], c0 * fr.x[0]));
         const Sorted so = sort3(p, q);
-        const cf em = expmi_rev(__builtin_fmaf(so.mid, kInv2PiF, reduce_rev(rev_a)));
-        cf H, dP, dQ;
-        face_H_grad(so, H, dP, dQ);
-        // Contract with conj(g) first: z = conj(g) e_mid, then
+        cf em, d, pP, pQ;
+        float rD;
+        if (so.D <= kWidePhase) {
+          em = expmi_rev(mid_rev<false>(rev_a, so.mid));
+          face_H_grad<false>(so, d, pP, pQ, rD);
+        } else {
+          em = expmi_rev(mid_rev<true>(rev_a, so.mid));
+          face_H_grad<true>(so, d, pP, pQ, rD);
+        }
+        // Contract with conj(g) first: z = conj(g) e_mid, then with H = d rD
         //   rho = Re(conj g F) = Re(z H),  gamma = Re(conj g (-iF)) = Im(z H),
-        //   Re(conj g dF/dx_lo) = Re(z dH/dP),  Re(conj g dF/dx_hi) = Re(z dH/dQ),
+        //   Re(conj g dF/dx_lo) = (Re(z pP) + rho) rD,
+        //   Re(conj g dF/dx_hi) = -(Re(z pQ) + rho) rD,
         //   Re(conj g dF/dx_mid) = gamma - (lo) - (hi).
         const cf z = cmul({fr.gr, -fr.gi}, em);
-        const float rh = z.re * H.re - z.im * H.im;
-        const float ga = z.re * H.im + z.im * H.re;
-        const float alo = z.re * dP.re - z.im * dP.im;
-        const float ahi = z.re * dQ.re - z.im * dQ.im;
+        const float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im) * rD;
+        const float ga = __builtin_fmaf(z.re, d.im, z.im * d.re) * rD;
+        const float alo = (__builtin_fmaf(z.re, pP.re, -z.im * pP.im) + rh) * rD;
+        const float ahi = -(__builtin_fmaf(z.re, pQ.re, -z.im * pQ.im) + rh) * rD;
         const float amid = ga - alo - ahi;
         const int ra = (p < 0.0f) + (q < 0.0f);
         const int rb = (p >= 0.0f) + (q < p);
         const float al = (ra == 0) ? alo : ((ra == 2) ? ahi : amid);
         const float be = (rb == 0) ? alo : ((rb == 2) ? ahi : amid);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          a[d] = __builtin_fmaf(al, fr.x[d], a[d]);
-          a[3 + d] = __builtin_fmaf(be, fr.x[d], a[3 + d]);
-          a[6 + d] = __builtin_fmaf(ga, fr.x[d], a[6 + d]);
+        for (int j = 0; j < 3; ++j) {
+          a[j] = __builtin_fmaf(al, fr.x[j], a[j]);
+          a[3 + j] = __builtin_fmaf(be, fr.x[j], a[3 + j]);
+          a[6 + j] = __builtin_fmaf(ga, fr.x[j], a[6 + j]);
         }
         a[9] += rh;
       }

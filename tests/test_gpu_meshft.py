@@ -151,6 +151,47 @@ def test_backward_cfg2_subset(device):
     assert err <= BWD_TOL
 
 
+def test_wide_phase_faces(device):
+    """Faces whose phase gap exceeds 1000 rad take the range-reduced variant
+    (their sin/cos arguments leave the hardware domain of 256 revolutions).
+    Vertices are multiples of 1/8 and frequencies multiples of 1/4, so every
+    FP32 edge vector and phase difference is exact and the usual tolerances
+    apply.  Large random triangles (phase gaps of thousands of rad) are mixed
+    with small ones (tens of rad) so both variants run in the same waves."""
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    g = syn.rng(77)
+    big = (g.integers(-48, 49, size=(120, 3)) / 8.0).astype(np.float32)
+    near = [(big[:32] + g.integers(-2, 3, size=(32, 3)) / 8.0).astype(np.float32) for _ in range(2)]
+    Vs = np.concatenate([big] + near)
+    faces = [g.choice(120, 3, replace=False) for _ in range(96)]
+    faces += [(i, 120 + i, 152 + i) for i in range(32)]
+    faces = np.array(faces, dtype=np.int64)
+    e1 = Vs[faces[:, 1]] - Vs[faces[:, 0]]
+    e2 = Vs[faces[:, 2]] - Vs[faces[:, 0]]
+    keep = np.linalg.norm(np.cross(e1, e2), axis=1) > 1e-3  # drop collinear draws
+    faces, e1, e2, nbig = faces[keep], e1[keep], e2[keep], int(keep[:96].sum())
+    coeff = g.uniform(0.5, 1.5, len(faces)).astype(np.float32)
+    xi = (g.integers(-1200, 1201, size=(768, 3)) / 4.0).astype(np.float32)
+    # phase gaps per pair: most large faces exceed the 1000 rad threshold
+    p, q = xi.astype(np.float64) @ e1.T, xi.astype(np.float64) @ e2.T
+    gap = np.maximum.reduce([np.abs(p), np.abs(q), np.abs(p - q)])
+    assert (gap[:, :nbig] > 1000).mean() > 0.5 and (gap[:, nbig:] < 1000).all()
+    V = torch.tensor(Vs, device=device)
+    Fc = torch.tensor(faces, device=device)
+    C = torch.tensor(coeff, device=device)
+    X = torch.tensor(xi, device=device)
+    ft = forward_fourier3d_cpp(V, Fc, C, X)
+    ref = O.acc_forward(Vs, faces, coeff, xi, threads=threads())
+    ferr = O.rel_l2(ft.cpu().numpy(), ref)
+    gr = _grad(len(xi), 78)
+    gv = backward_fourier3d_cpp(V, Fc, C, torch.tensor(gr, device=device), ft, X)
+    refb = O.acc_backward(Vs, faces, coeff, xi, gr, threads=threads())
+    berr = O.rel_l2(gv.cpu().numpy(), refb)
+    print(f"wide-phase faces: forward rel-L2 {ferr:.3e}, adjoint rel-L2 {berr:.3e}")
+    assert ferr <= FWD_TOL
+    assert berr <= BWD_TOL
+
+
 def test_bitwise_deterministic(device):
     from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
     w = syn.config1()
