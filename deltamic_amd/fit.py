@@ -143,10 +143,25 @@ class FitResult:
     accepted_at: list = field(default_factory=list)  # learned fit: iterations that took a proposal
 
 
+def adam_direction(opt: torch.optim.Adam, p: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
+    """The normalised step m_hat / (sqrt(v_hat) + eps) that ``opt`` (torch.optim.Adam,
+    one parameter group) would take next for parameter ``p`` with gradient ``g``,
+    without the learning rate and without changing the optimiser state."""
+    group = opt.param_groups[0]
+    b1, b2 = group["betas"]
+    eps = group["eps"]
+    st = opt.state.get(p, {})
+    t = int(st["step"]) + 1 if "step" in st else 1
+    m = st["exp_avg"] * b1 + (1 - b1) * g if "exp_avg" in st else (1 - b1) * g
+    v = st["exp_avg_sq"] * b2 + (1 - b2) * g * g if "exp_avg_sq" in st else (1 - b2) * g * g
+    return (m / (1 - b1 ** t)) / ((v / (1 - b2 ** t)).sqrt() + eps)
+
+
 def fit_adam(problem: FitProblem, steps: int = 1000, lr: float = 1e-3, snapshot_every: int = 0,
              sync_every: int = 1, stop_loss: float | None = None) -> FitResult:
     """Plain Adam on the vertices (config 4).  ``losses[k]`` is the loss at the
-    iterate before step k; the last entry is the loss after the final step."""
+    iterate before step k; the last entry is the loss after the final step.
+    Every ``snapshot_every`` steps (vertices, gradient, Adam direction) is kept."""
     V = problem.init_verts.clone().requires_grad_(True)
     opt = torch.optim.Adam([V], lr=lr)
     losses, times, snaps = [], [], []
@@ -157,7 +172,8 @@ def fit_adam(problem: FitProblem, steps: int = 1000, lr: float = 1e-3, snapshot_
         L = problem.loss(V)
         L.backward()
         if snapshot_every and k % snapshot_every == 0:
-            snaps.append((k, V.detach().clone(), V.grad.detach().clone()))
+            snaps.append((k, V.detach().clone(), V.grad.detach().clone(),
+                          adam_direction(opt, V, V.grad.detach())))
         opt.step()
         losses.append(L.detach())
         if sync_every and (k + 1) % sync_every == 0:

@@ -5,16 +5,18 @@ bf16 weights, FP32 accumulation; HIP kernels in csrc/mlp.hip) maps
 per-vertex features
 
     position (centred, scaled by the mesh RMS radius), unit vertex normal,
-    loss gradient and its 1-ring mean (both multiplied by nv * grad_scale:
-    per-vertex gradients scale like the vertex area ~ 1/nv, and grad_scale
-    is calibrated on the training trajectories, so the features keep the
-    absolute gradient size the step length depends on)
+    loss gradient (multiplied by nv * grad_scale: per-vertex gradients scale
+    like the vertex area ~ 1/nv, and grad_scale is calibrated on the training
+    trajectories, so the feature keeps the absolute gradient size the step
+    length depends on), and the normalised step m_hat / (sqrt(v_hat) + eps)
+    that Adam would take next
 
-to a displacement.  The network is trained on the method's own plain-Adam
-fitting trajectories: the target for a snapshot (V_k, grad_k) is the
-displacement Adam makes over the next ``horizon`` steps.  In the fit, a
-proposal V + d is accepted only if the meshFT + OTF loss decreases; otherwise
-a plain Adam step is taken (fit_learned).
+to a displacement, which is smoothed over the 1-ring before use.  The
+network is trained on the method's own plain-Adam fitting trajectories: the
+target for a snapshot (V_k, grad_k, Adam state_k) is the displacement Adam
+makes over the next ``horizon`` steps.  In the fit, a proposal V + d is
+accepted only if the meshFT + OTF loss decreases by more than the Adam step
+would decrease it; otherwise a plain Adam step is taken (fit_learned).
 """
 from __future__ import annotations
 
@@ -26,7 +28,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .fit import FitProblem, FitResult
+from .fit import FitProblem, FitResult, adam_direction
 
 N_IN, N_HID, N_OUT = 12, 128, 3
 
@@ -60,7 +62,7 @@ class Features:
         self.grad_scale = grad_scale
 
     @torch.no_grad()
-    def __call__(self, V: torch.Tensor, grad: torch.Tensor) -> torch.Tensor:
+    def __call__(self, V: torch.Tensor, grad: torch.Tensor, adir: torch.Tensor) -> torch.Tensor:
         c = V.mean(0, keepdim=True)
         P = V - c
         rad = P.pow(2).sum(1).mean().sqrt().clamp_min(1e-12)
@@ -72,9 +74,15 @@ class Features:
             g = grad / grad.pow(2).sum(1).mean().sqrt().clamp_min(1e-30)
         else:
             g = grad * (self.nv * self.grad_scale)
+        return torch.cat([P / rad, vn, g, adir], dim=1).contiguous()
+
+    @torch.no_grad()
+    def smooth(self, d: torch.Tensor, passes: int = 2) -> torch.Tensor:
+        """Each pass replaces a displacement by the mean of itself and its 1-ring mean."""
         lap = self.lap
-        gm = (g[lap.nbr] * lap.mask[..., None]).sum(1) * lap.invdeg[:, None]
-        return torch.cat([P / rad, vn, g, gm], dim=1).contiguous()
+        for _ in range(passes):
+            d = 0.5 * (d + (d[lap.nbr] * lap.mask[..., None]).sum(1) * lap.invdeg[:, None])
+        return d
 
 
 class ProposalMLP:
@@ -185,46 +193,57 @@ def trajectory_samples(problem: FitProblem, feats: Features, steps: int, every: 
     """Runs a plain Adam fit and turns its snapshots into (features, displacement) pairs."""
     from .fit import fit_adam
     res = fit_adam(problem, steps=steps, lr=lr, snapshot_every=every, sync_every=0)
-    snaps = res.snapshots + [(steps, res.verts, None)]
-    by_step = {k: v for k, v, _ in snaps}
+    by_step = {k: V for k, V, _, _ in res.snapshots}
+    by_step[steps] = res.verts
     xs, ts = [], []
-    for k, V, g in res.snapshots:
+    for k, V, g, adir in res.snapshots:
         k2 = min(k + horizon, steps)
         if k2 == k or k2 not in by_step:
             continue
-        xs.append(feats(V, g))
+        xs.append(feats(V, g, adir))
         ts.append(by_step[k2] - V)
     return TrajectoryData(torch.cat(xs), torch.cat(ts))
 
 
 # ------------------------------------------------------------------ learned fit
-def _adam_predicted_decrease(opt: torch.optim.Adam, p: torch.Tensor, g: torch.Tensor) -> float:
+def _adam_predicted_decrease(opt: torch.optim.Adam, p: torch.Tensor, g: torch.Tensor,
+                             adir: torch.Tensor) -> float:
     """First-order loss change g . delta of the Adam step opt would take with gradient g."""
+    return float(-(g * adir).sum()) * opt.param_groups[0]["lr"]
+
+
+@torch.no_grad()
+def _adam_absorb(opt: torch.optim.Adam, p: torch.Tensor, g: torch.Tensor) -> None:
+    """Advance Adam's moment estimates with gradient g without moving p (the
+    iteration's step was an accepted proposal instead)."""
     group = opt.param_groups[0]
     b1, b2 = group["betas"]
-    lr, eps = group["lr"], group["eps"]
-    st = opt.state.get(p, {})
-    t = int(st.get("step", 0)) + 1
-    m = st["exp_avg"] * b1 + (1 - b1) * g if "exp_avg" in st else (1 - b1) * g
-    v = st["exp_avg_sq"] * b2 + (1 - b2) * g * g if "exp_avg_sq" in st else (1 - b2) * g * g
-    step = lr * (m / (1 - b1 ** t)) / ((v / (1 - b2 ** t)).sqrt() + eps)
-    return float(-(g * step).sum())
+    st = opt.state[p]
+    if not st:
+        st["step"] = torch.tensor(0.0)
+        st["exp_avg"] = torch.zeros_like(p, memory_format=torch.preserve_format)
+        st["exp_avg_sq"] = torch.zeros_like(p, memory_format=torch.preserve_format)
+    st["step"] += 1
+    st["exp_avg"].mul_(b1).add_(g, alpha=1 - b1)
+    st["exp_avg_sq"].mul_(b2).addcmul_(g, g, value=1 - b2)
 
 
 def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: int = 1000,
                 lr: float = 1e-3, stop_loss: float | None = None, scale0: float = 1.0,
                 grow: float = 1.5, shrink: float = 0.5, scale_max: float = 8.0,
-                patience: int = 4, cooldown: int = 8) -> FitResult:
-    """Accept/reject fit.  Each iteration the network proposes V + s*d (trust
-    scale s grows after an accepted proposal, shrinks after a rejection).
-    The candidate's loss is computed with its autograd graph and accepted only
-    if it is lower than the current loss by more than the first-order
-    decrease of the Adam step that would otherwise be taken; its gradient then
-    comes from that same graph (one forward + one adjoint, the cost of an Adam
-    step).  Otherwise the candidate is discarded and the plain Adam step is
-    taken.  After ``patience`` rejections in a row no proposal is made for
-    ``cooldown`` iterations (plain Adam).  ``losses[k]`` is the loss after
-    iteration k."""
+                patience: int = 4, cooldown: int = 8, cooldown_max: int = 256) -> FitResult:
+    """Accept/reject fit.  Each iteration the network proposes V + s*d (d
+    smoothed over the 1-ring; trust scale s grows after an accepted proposal,
+    shrinks after a rejection).  The candidate's loss is computed with its
+    autograd graph and accepted only if it is lower than the current loss by
+    more than the first-order decrease of the Adam step that would otherwise
+    be taken; its gradient then comes from that same graph (one forward + one
+    adjoint, the cost of an Adam step) and Adam's moments absorb the gradient
+    of the point left behind.  Otherwise the candidate is discarded and the
+    plain Adam step is taken.  After ``patience`` rejections in a row no
+    proposal is made for ``cooldown`` iterations, a pause that doubles (up to
+    ``cooldown_max``) each time it recurs without an acceptance in between.
+    ``losses[k]`` is the loss after iteration k."""
     Vp = problem.init_verts.clone().requires_grad_(True)
     opt = torch.optim.Adam([Vp], lr=lr)
     losses, times = [], []
@@ -233,6 +252,7 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
     s = scale0
     misses = 0
     quiet = 0
+    pause = cooldown
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     L, g = problem.loss_and_grad(Vp)
@@ -241,14 +261,15 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
         if quiet > 0:
             quiet -= 1
         else:
-            x = feats(Vp.detach(), g)
-            d = net.propose(x)
+            adir = adam_direction(opt, Vp, g)
+            d = feats.smooth(net.propose(feats(Vp.detach(), g, adir)))
             proposals += 1
             cand = (Vp.detach() + s * d).requires_grad_(True)
             Lc = problem.loss(cand)
-            bar = float(L) + _adam_predicted_decrease(opt, Vp, g)
+            bar = float(L) + _adam_predicted_decrease(opt, Vp, g, adir)
             if float(Lc.detach()) < bar:
                 (gc,) = torch.autograd.grad(Lc, cand)
+                _adam_absorb(opt, Vp, g)
                 with torch.no_grad():
                     Vp.copy_(cand.detach())
                 L, g = Lc.detach(), gc
@@ -256,13 +277,15 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
                 accepted_at.append(k)
                 took = True
                 misses = 0
+                pause = cooldown
                 s = min(s * grow, scale_max)
             else:
                 del Lc, cand
                 s = max(s * shrink, 1.0 / 64)
                 misses += 1
                 if misses >= patience:
-                    quiet, misses = cooldown, 0
+                    quiet, misses = pause, 0
+                    pause = min(2 * pause, cooldown_max)
         if not took:
             Vp.grad = g
             opt.step()

@@ -192,7 +192,8 @@ def test_uniform_laplacian_adjoint_matches_dense_autograd():
 
 def test_proposal_features_on_a_sphere():
     """Per-vertex features: centred/scaled position, outward unit normal,
-    gradient scaled to unit RMS and its 1-ring mean."""
+    gradient scaled to unit RMS, Adam direction passed through; 1-ring
+    smoothing of a displacement field."""
     from deltamic_amd.fit import UniformLaplacian
     from deltamic_amd.proposal import Features
     u, f = syn.icosphere(3)
@@ -200,13 +201,43 @@ def test_proposal_features_on_a_sphere():
     lap = UniformLaplacian(f, len(u), "cpu")
     feats = Features(f, len(u), lap, "cpu")
     g = torch.tensor(syn.rng(3).normal(size=u.shape), dtype=torch.float32)
-    x = feats(V, 7.0 * g)
+    adir = torch.tensor(syn.rng(4).normal(size=u.shape), dtype=torch.float32)
+    x = feats(V, 7.0 * g, adir)
     assert x.shape == (len(u), 12)
-    P, n, gg, gm = x[:, 0:3], x[:, 3:6], x[:, 6:9], x[:, 9:12]
+    P, n, gg, ad = x[:, 0:3], x[:, 3:6], x[:, 6:9], x[:, 9:12]
     assert torch.allclose(P.pow(2).sum(1).mean(), torch.tensor(1.0), atol=1e-5)
     assert torch.allclose(n.norm(dim=1), torch.ones(len(u)), atol=1e-5)
     assert (torch.sum(n * torch.tensor(u, dtype=torch.float32), 1) > 0.99).all()  # outward
     assert torch.allclose(gg.pow(2).sum(1).mean(), torch.tensor(1.0), atol=1e-5)
+    assert torch.equal(ad, adir)
+    # smoothing: one pass is the mean of a value and its 1-ring mean; constants are fixed
+    d1 = feats.smooth(g, passes=1)
     i = 17
     nb = lap.nbr[i][lap.mask[i] > 0]
-    assert torch.allclose(gm[i], gg[nb].mean(0), atol=1e-6)
+    assert torch.allclose(d1[i], 0.5 * (g[i] + g[nb].mean(0)), atol=1e-6)
+    c = torch.ones_like(g) * torch.tensor([0.3, -1.0, 2.0])
+    assert torch.allclose(feats.smooth(c), c, atol=1e-6)
+    assert feats.smooth(g).norm() < 0.8 * g.norm()
+
+
+def test_adam_direction_matches_torch_step():
+    """adam_direction predicts torch.optim.Adam's next step exactly, and
+    absorbing a gradient advances the moments as a step would."""
+    from deltamic_amd.fit import adam_direction
+    from deltamic_amd.proposal import _adam_absorb
+    gen = syn.rng(5)
+    p = torch.tensor(gen.normal(size=(40, 3)), dtype=torch.float32, requires_grad=True)
+    q = p.detach().clone().requires_grad_(True)
+    opt = torch.optim.Adam([p], lr=1e-2)
+    opt_q = torch.optim.Adam([q], lr=1e-2)
+    for it in range(4):
+        g = torch.tensor(gen.normal(size=(40, 3)), dtype=torch.float32)
+        pred = -1e-2 * adam_direction(opt, p, g)
+        before = p.detach().clone()
+        p.grad = g
+        opt.step()
+        assert torch.allclose(p.detach() - before, pred, atol=1e-7)
+        _adam_absorb(opt_q, q, g)
+    st, sq = opt.state[p], opt_q.state[q]
+    assert int(st["step"]) == int(sq["step"]) == 4
+    assert torch.allclose(st["exp_avg"], sq["exp_avg"]) and torch.allclose(st["exp_avg_sq"], sq["exp_avg_sq"])
