@@ -62,6 +62,52 @@ def cpu_baseline(w, xi_np, Vs, n_sample: int):
     }
 
 
+def psf_leg(L, dev, w, steps):
+    """Learnable Gaussian PSF on the bench box: forward (writes 4 B/voxel) and
+    sigma gradient (reads 8 B/voxel), timed with events on the launch stream."""
+    import torch
+    from deltamic_amd import _lib
+    n0, n1, n2 = (int(x) for x in w.box_shape)
+    axes = [torch.linspace(-1.0, 1.0, n, device=dev) for n in (n0, n1, n2)]
+    S = (1e3 * torch.eye(3, device=dev)).contiguous()
+    out = torch.empty((n0, n1, n2), device=dev)
+    G = torch.ones_like(out)
+    gS = torch.empty(9, device=dev)
+    nbytes = int(L.dmic_psf_grad_workspace(n0, n1, n2))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    st = _lib.stream_of(out)
+    ap = [a.data_ptr() for a in axes]
+
+    def fwd():
+        _lib.check(L.dmic_psf_gaussian(S.data_ptr(), *ap, n0, n1, n2, out.data_ptr(), st),
+                   "dmic_psf_gaussian")
+
+    def grad():
+        _lib.check(L.dmic_psf_gaussian_grad(G.data_ptr(), out.data_ptr(), *ap, n0, n1, n2,
+                                            gS.data_ptr(), ws.data_ptr(), nbytes, st),
+                   "dmic_psf_gaussian_grad")
+
+    res = {}
+    for name, fn, nbytes_moved in (("forward", fwd, 4 * n0 * n1 * n2),
+                                   ("grad", grad, 8 * n0 * n1 * n2)):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        res[f"{name}_ms"] = round(ms, 4)
+        res[f"{name}_GBps"] = round(nbytes_moved / (ms * 1e-3) / 1e9, 1)
+    res["bytes_forward"] = 4 * n0 * n1 * n2
+    res["bytes_grad"] = 8 * n0 * n1 * n2
+    res["hbm_peak_GBps"] = 8000.0
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +124,9 @@ def main() -> int:
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--module-steps", type=int, default=3,
                     help="extra end-to-end module steps (forward, render, backward) to time")
+    ap.add_argument("--psf-steps", type=int, default=5,
+                    help="also time the learnable-PSF kernels (forward and sigma gradient) on "
+                         "the box (HBM-bound; reported separately under 'psf')")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -232,6 +281,13 @@ def main() -> int:
         module_ms = a.elapsed_time(b) / args.module_steps
         log(f"module step (fwd+render+normalize+bwd): {module_ms:.3f} ms")
 
+    psf = None
+    if args.psf_steps > 0:
+        psf = psf_leg(L, dev, w, args.psf_steps)
+        log(f"psf kernels on {w.box_shape}: forward {psf['forward_ms']:.4f} ms "
+            f"({psf['forward_GBps']:.0f} GB/s), sigma gradient {psf['grad_ms']:.4f} ms "
+            f"({psf['grad_GBps']:.0f} GB/s)")
+
     cpu = None
     if args.cpu_freqs > 0:
         log("cpu baseline ...")
@@ -290,6 +346,7 @@ def main() -> int:
             "module_step_ms": None if module_ms is None else round(module_ms, 3),
         },
         "full_band_evaluation": full_band,
+        "psf": psf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

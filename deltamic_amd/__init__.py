@@ -4,13 +4,17 @@ hand-written HIP kernels for AMD Instinct MI355X (gfx950).
 Public names follow the reference's ``deltamic`` package (``from .psf import *``
 and ``from .render import *`` in deltamic/__init__.py:1-2).
 """
-from .psf import generate_gaussian_psf
+from .psf import (Anisotropic_gaussian_psf, Gaussian_psf, Gaussian_psf_logsigma,
+                  generate_gaussian_psf)
 from .render import (Fourier3dMesh, compute_box_size, normalize_tensor,
                      render_image_from_ftmesh, spatial_frequency_axes)
 from .hip_class import MeshFTHIP, MeshFTCUDA
 
 __all__ = [
+    "Anisotropic_gaussian_psf",
     "Fourier3dMesh",
+    "Gaussian_psf",
+    "Gaussian_psf_logsigma",
     "MeshFTCUDA",
     "MeshFTHIP",
     "compute_box_size",

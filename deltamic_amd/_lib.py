@@ -44,6 +44,13 @@ SIGNATURES = {
     "dmic_band_mirror_fill": ([_vp, _i64, _vp], _int),
     "dmic_band_mirror_fold": ([_vp, _i64, _vp, _vp], _int),
     "dmic_gaussian_otf": ([_vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp], _int),
+    "dmic_psf_gaussian": ([_vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp], _int),
+    "dmic_psf_grad_workspace": ([_i64, _i64, _i64], _sz),
+    "dmic_psf_gaussian_grad": ([_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _sz, _vp],
+                               _int),
+    "dmic_psf_aniso": ([_vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp], _int),
+    "dmic_psf_aniso_grad": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _sz,
+                             _vp], _int),
     "dmic_gemm_bf16": ([_vp, _int, _i64, _i64, _vp, _int, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
                         _vp, _vp, _i64, _int, _vp], _int),
     "dmic_mlp_weight_count": ([], _i64),

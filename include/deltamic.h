@@ -158,6 +158,40 @@ int dmic_gaussian_otf(const float* sigma_host, const float* xi0, const float* xi
                       const float* xi2, int64_t n0, int64_t n1, int64_t n2, float* out,
                       hipStream_t stream);
 
+/* ------------------------------------------------- learnable PSFs
+ *
+ * Replace the forward of Gaussian_psf / Gaussian_psf_logsigma
+ * (deltamic/psf.py:551-625) and Anisotropic_gaussian_psf (psf.py:628-665)
+ * and the autograd backward torch derives for them.  Parameters live in
+ * DEVICE memory (they are learnable tensors).  Axes x0[n0], x1[n1], x2[n2];
+ * the PSF is [n0][n1][n2]; n0 and n1 must be <= 65535.
+ */
+
+/* psf = exp(-x^T S x / 2); S: 9 floats, row-major 3x3 (psf.py:572-576). */
+int dmic_psf_gaussian(const float* S, const float* x0, const float* x1, const float* x2,
+                      int64_t n0, int64_t n1, int64_t n2, float* out, hipStream_t stream);
+
+/* Workspace bytes for the two gradient entry points below. */
+size_t dmic_psf_grad_workspace(int64_t n0, int64_t n1, int64_t n2);
+
+/* grad_S[9] (device) = dL/dS given grad_out = dL/dpsf and the psf of the
+ * forward.  Fixed-order FP64 reduction: bitwise reproducible. */
+int dmic_psf_gaussian_grad(const float* grad_out, const float* psf, const float* x0,
+                           const float* x1, const float* x2, int64_t n0, int64_t n1, int64_t n2,
+                           float* grad_S, void* workspace, size_t workspace_bytes,
+                           hipStream_t stream);
+
+/* psf = exp(-Q / (2D)) / D, Q = (x0, x1) S2 (x0, x1)^T, D = eps + (|x2| sz)^2
+ * (psf.py:655-660); prm = {S2_00, S2_01, S2_10, S2_11, sz, eps}. */
+int dmic_psf_aniso(const float* prm, const float* x0, const float* x1, const float* x2,
+                   int64_t n0, int64_t n1, int64_t n2, float* out, hipStream_t stream);
+
+/* grad_prm[6] (device) = dL/dprm, same layout as prm. */
+int dmic_psf_aniso_grad(const float* prm, const float* grad_out, const float* psf,
+                        const float* x0, const float* x1, const float* x2, int64_t n0, int64_t n1,
+                        int64_t n2, float* grad_prm, void* workspace, size_t workspace_bytes,
+                        hipStream_t stream);
+
 /* ------------------------------------------------- proposal network (MLP)
  *
  * No counterpart in the reference: the learned per-vertex displacement

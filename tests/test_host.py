@@ -150,13 +150,22 @@ def test_native_boundary_checks_like_reference():
                                torch.zeros(4, dtype=torch.complex64), xi)
 
 
-def test_gaussian_psf_cpu_matches_restatement():
-    from deltamic_amd import generate_gaussian_psf
+def test_psf_on_cpu_raises():
+    """No CPU path: the OTF generator and the learnable PSF modules refuse CPU
+    tensors, as the meshFT module does (their GPU numerics are in
+    tests/test_gpu_psf.py)."""
+    from deltamic_amd import Anisotropic_gaussian_psf, Gaussian_psf, generate_gaussian_psf
     ax = [torch.linspace(-30, 30, n) for n in (5, 6, 7)]
     S = torch.tensor([[1e-3, 2e-4, 0.0], [2e-4, 2e-3, 1e-4], [0.0, 1e-4, 4e-3]])
-    got = generate_gaussian_psf(S, *ax).numpy()
-    want = O.gaussian_otf(S.numpy(), *[a.numpy() for a in ax])
-    assert np.allclose(got, want, rtol=1e-5)
+    with pytest.raises(NotImplementedError):
+        generate_gaussian_psf(S, *ax)
+    with pytest.raises(NotImplementedError):
+        Gaussian_psf((4, 5, 6), torch.tensor([1.0, 1.0, 1.0]), sigma=10.0)()
+    with pytest.raises(NotImplementedError):
+        Anisotropic_gaussian_psf((4, 5, 6), [1.0, 1.0, 1.0])()
+    m = Gaussian_psf((4, 5, 6), torch.tensor([1.0, 2.0, 3.0]), sigma=10.0)
+    assert torch.equal(m.sigma.detach(), 10.0 * torch.eye(3))
+    assert m.x2[0] == -1.5 and m.x2[-1] == 1.5 and m.Gridxi1.shape == (4, 5, 6)
 
 
 def test_bench_refuses_more_than_one_gpu():
