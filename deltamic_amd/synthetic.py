@@ -186,6 +186,31 @@ def config4(k: int = 0, subdivisions: int = 4, box: int = 128) -> Workload:
                     extra={"init_verts": u.astype(np.float32)})
 
 
+def doublet(subdivisions: int = 3, seed: int = 6000):
+    """A two-cell multi-material mesh in the ``.rec`` labelling: an icosphere
+    (jittered by N(0, 0.003)) cut by the plane z = 0 into cell 1 (face
+    centroids above) and cell 2 (below), whose exterior faces carry labels
+    (1, 0) / (2, 0), plus an interface disc fanned from a centre vertex to the
+    loop of edges where the two halves meet, labelled (1, 2).  Returns
+    (verts [nv, 3] float64, faces_mm [nf, 5] int64)."""
+    u, f = icosphere(subdivisions)
+    v = u + rng(seed).normal(0.0, 0.003, size=u.shape)
+    top = v[f].mean(1)[:, 2] > 0.0
+    labels = np.where(top, 1, 2)
+    # boundary loop: directed edges of the top faces whose reverse is a bottom-face edge
+    top_edges = {(a, b) for tri in f[top] for a, b in ((tri[0], tri[1]), (tri[1], tri[2]),
+                                                        (tri[2], tri[0]))}
+    bot_edges = {(a, b) for tri in f[~top] for a, b in ((tri[0], tri[1]), (tri[1], tri[2]),
+                                                         (tri[2], tri[0]))}
+    loop = sorted((a, b) for a, b in top_edges if (b, a) in bot_edges)
+    centre = len(v)
+    loop_verts = np.unique(np.array(loop).ravel())
+    v = np.vstack([v, [[0.0, 0.0, float(v[loop_verts, 2].mean())]]])
+    faces = [list(tri) + [int(lab), 0] for tri, lab in zip(f, labels)]
+    faces += [[a, b, centre, 1, 2] for a, b in loop]
+    return v, np.array(faces, dtype=np.int64)
+
+
 def noisy_image(img: np.ndarray, seed: int, photons: float = 500.0, sigma: float = 0.01):
     """Poisson(photons * img) / photons + N(0, sigma^2) (config 4's noise model)."""
     g = rng(seed)
