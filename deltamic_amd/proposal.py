@@ -242,7 +242,8 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
     of the point left behind.  Otherwise the candidate is discarded and the
     plain Adam step is taken.  After ``patience`` rejections in a row no
     proposal is made for ``cooldown`` iterations, a pause that doubles (up to
-    ``cooldown_max``) each time it recurs without an acceptance in between.
+    ``cooldown_max``) each time it recurs without an acceptance in between;
+    the trust scale restarts from ``scale0`` after each pause.
     ``losses[k]`` is the loss after iteration k."""
     Vp = problem.init_verts.clone().requires_grad_(True)
     opt = torch.optim.Adam([Vp], lr=lr)
@@ -286,6 +287,7 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
                 if misses >= patience:
                     quiet, misses = pause, 0
                     pause = min(2 * pause, cooldown_max)
+                    s = scale0  # the next proposal starts again from the full step
         if not took:
             Vp.grad = g
             opt.step()
