@@ -187,3 +187,31 @@ def fit_adam(problem: FitProblem, steps: int = 1000, lr: float = 1e-3, snapshot_
     times.append(time.perf_counter() - t0)
     return FitResult(verts=V.detach(), losses=[float(x) for x in losses], times=times,
                      snapshots=snaps)
+
+
+# ------------------------------------------------------------------ large steps
+def fit_largesteps(problem: FitProblem, steps: int = 100, lr: float = 1e-2,
+                   lambda_: float = 50.0) -> FitResult:
+    """The reference's preconditioned fit (Examples/Gradient_Descent.ipynb
+    cell 2) on this problem's loss: AdamUniform on the vertices with the
+    gradient replaced by (I + lambda L)^-2 grad (deltamic_amd.optim)."""
+    from .optim import AdamUniform, LargeStepsPreconditioner
+    V = problem.init_verts.clone().requires_grad_(True)
+    pre = LargeStepsPreconditioner(problem.faces.cpu().numpy(), V.shape[0], lambda_, V.device)
+    opt = AdamUniform([V], lr=lr)
+    losses, times = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        opt.zero_grad(set_to_none=True)
+        L = problem.loss(V)
+        L.backward()
+        with torch.no_grad():
+            V.grad = pre.solve(V.grad)
+        opt.step()
+        losses.append(float(L))
+        times.append(time.perf_counter() - t0)
+    with torch.no_grad():
+        losses.append(float(problem.loss(V)))
+    times.append(time.perf_counter() - t0)
+    return FitResult(verts=V.detach(), losses=losses, times=times)

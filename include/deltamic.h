@@ -164,7 +164,7 @@ int dmic_gaussian_otf(const float* sigma_host, const float* xi0, const float* xi
  * (deltamic/psf.py:551-625) and Anisotropic_gaussian_psf (psf.py:628-665)
  * and the autograd backward torch derives for them.  Parameters live in
  * DEVICE memory (they are learnable tensors).  Axes x0[n0], x1[n1], x2[n2];
- * the PSF is [n0][n1][n2]; n0 and n1 must be <= 65535.
+ * the PSF is [n0][n1][n2].
  */
 
 /* psf = exp(-x^T S x / 2); S: 9 floats, row-major 3x3 (psf.py:572-576). */
