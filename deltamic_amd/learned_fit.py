@@ -108,7 +108,8 @@ def main(argv=None):
     ap.add_argument("--train-fits", type=int, default=24)
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--every", type=int, default=10)
-    ap.add_argument("--horizon", type=int, default=100)
+    ap.add_argument("--horizon", type=int, default=100,
+                    help="displacement target over this many steps (<= 0: to the end of the fit)")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -119,6 +120,7 @@ def main(argv=None):
     ap.add_argument("--box", type=int, default=128)
     ap.add_argument("--out", default="gpurun_out/learned")
     ap.add_argument("--load", default=None, help="skip training; load proposal.pt")
+    ap.add_argument("--no-eval", action="store_true", help="train and save only")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
     device = torch.device("cuda:0")
@@ -137,6 +139,10 @@ def main(argv=None):
         summary["train_loss"] = train(net, x, t, args.epochs, args.batch, args.lr, seed=1)
         summary["train_s"] = time.perf_counter() - t1
         torch.save(net.state_dict(), os.path.join(args.out, "proposal.pt"))
+    with open(os.path.join(args.out, "summary.json"), "w") as f:
+        json.dump(summary, f, indent=1)
+    if args.no_eval:
+        return
     evals = []
     jobs = [(args.subdiv, k) for k in range(args.eval_shapes)]
     jobs += [(args.subdiv + 1, 100 + k) for k in range(args.eval_shapes_large)]

@@ -190,14 +190,16 @@ class TrajectoryData:
 
 def trajectory_samples(problem: FitProblem, feats: Features, steps: int, every: int,
                        horizon: int, lr: float = 1e-3) -> TrajectoryData:
-    """Runs a plain Adam fit and turns its snapshots into (features, displacement) pairs."""
+    """Runs a plain Adam fit and turns its snapshots into (features, displacement)
+    pairs: the displacement over the next ``horizon`` steps, or to the end of
+    the fit when ``horizon`` <= 0."""
     from .fit import fit_adam
     res = fit_adam(problem, steps=steps, lr=lr, snapshot_every=every, sync_every=0)
     by_step = {k: V for k, V, _, _ in res.snapshots}
     by_step[steps] = res.verts
     xs, ts = [], []
     for k, V, g, adir in res.snapshots:
-        k2 = min(k + horizon, steps)
+        k2 = steps if horizon <= 0 else min(k + horizon, steps)
         if k2 == k or k2 not in by_step:
             continue
         xs.append(feats(V, g, adir))
