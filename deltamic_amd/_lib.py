@@ -31,6 +31,11 @@ SIGNATURES = {
     "dmic_meshft_backward_workspace": ([_i64, _i64, _i64], _sz),
     "dmic_meshft_backward": (
         [_vp, _i64, _vp, _int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz, _vp], _int),
+    "dmic_meshft_af_forward": (
+        [_vp, _i64, _vp, _int, _i64, _vp, _f32, _vp, _i64, _vp, _vp, _sz, _vp], _int),
+    "dmic_meshft_af_backward_workspace": ([_i64, _i64, _i64], _sz),
+    "dmic_meshft_af_backward": (
+        [_vp, _i64, _vp, _int, _i64, _vp, _f32, _vp, _vp, _i64, _vp, _vp, _sz, _vp], _int),
     "dmic_check_index_flag": ([_vp, _vp], _int),
     "dmic_profile_enable": ([_int], _int),
     "dmic_profile_read": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long),

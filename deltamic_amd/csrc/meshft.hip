@@ -42,7 +42,7 @@ struct __align__(16) FaceRec {
   float eb[3];   // v_b - v_a (FP32, correctly rounded)
   float ec[3];   // v_c - v_a
   float w;       // 2 c_t A_t / sum c A
-  float pad[3];
+  float an[3];   // autofluorescence variant: (regint / regvol) (v_b - v_a) x (v_c - v_a); else 0
 };
 static_assert(sizeof(FaceRec) == 64, "FaceRec layout");
 
@@ -51,7 +51,7 @@ struct __align__(16) FreqRec {
   double X[3];  // xi in FP64
   float x[3];   // xi in FP32
   float gr, gi; // upstream gradient g = dL/dRe + i dL/dIm
-  float pad;
+  float inv2;   // autofluorescence variant: 1 / |xi|^2 (0 at xi = 0); else 0
 };
 static_assert(sizeof(FreqRec) == 48, "FreqRec layout");
 
@@ -85,15 +85,18 @@ __device__ __forceinline__ void load_face_verts(const float* __restrict__ V, int
   }
 }
 
-// c_t A_t (FP64, cross-product norm as .cu:9-12,38-40) and the fixed-order
-// sum S = sum_t c_t A_t, in one workgroup.
+// c_t A_t (FP64, cross-product norm as .cu:9-12,38-40) and, in one
+// workgroup and a fixed order, S[0] = sum_t c_t A_t and
+// S[1] = regvol = -sum_t a . (b x c) / 6 (compute_volume_manifold,
+// render.py:221-226; used by the autofluorescence variant).
 template <typename IdxT>
 __global__ __launch_bounds__(1024) void face_area_sum_kernel(
     const float* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
     const float* __restrict__ coeff, double* __restrict__ cA, double* __restrict__ S_out,
     int* flag) {
   __shared__ double red[1024];
-  double s = 0.0;
+  __shared__ double redv[1024];
+  double s = 0.0, vs = 0.0;
   for (int64_t t = threadIdx.x; t < nf; t += 1024) {
     float P[3][3];
     load_face_verts(V, nv, Fc, t, P, flag);
@@ -109,27 +112,39 @@ __global__ __launch_bounds__(1024) void face_area_sum_kernel(
     const double ca = (double)coeff[t] * 0.5 * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
     cA[t] = ca;
     s += ca;
+    const double p0 = P[0][0], p1 = P[0][1], p2 = P[0][2];
+    vs += p0 * ((double)P[1][1] * P[2][2] - (double)P[1][2] * P[2][1]) +
+          p1 * ((double)P[1][2] * P[2][0] - (double)P[1][0] * P[2][2]) +
+          p2 * ((double)P[1][0] * P[2][1] - (double)P[1][1] * P[2][0]);
   }
   red[threadIdx.x] = s;
+  redv[threadIdx.x] = vs;
   __syncthreads();
   for (int w = 512; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+      redv[threadIdx.x] += redv[threadIdx.x + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) S_out[0] = red[0];
+  if (threadIdx.x == 0) {
+    S_out[0] = red[0];
+    S_out[1] = -redv[0] / 6.0;
+  }
 }
 
 template <typename IdxT>
 __global__ __launch_bounds__(256) void face_pack_kernel(
     const float* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
-    const double* __restrict__ cA, const double* __restrict__ S_in, FaceRec* __restrict__ out,
-    int* flag) {
+    const double* __restrict__ cA, const double* __restrict__ S_in, float regint,
+    FaceRec* __restrict__ out, int* flag) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= nf) return;
   float P[3][3];
   load_face_verts(V, nv, Fc, t, P, flag);
   FaceRec r;
   const double S = S_in[0];
+  const double regvol = S_in[1];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     r.a[d] = (double)P[0][d] * kInv2Pi;
@@ -137,16 +152,33 @@ __global__ __launch_bounds__(256) void face_pack_kernel(
     r.ec[d] = P[2][d] - P[0][d];
   }
   r.w = (S > 0.0) ? (float)(2.0 * cA[t] / S) : 0.0f;
-  r.pad[0] = r.pad[1] = r.pad[2] = 0.0f;
+  // autofluorescence: nor2_t = nor_t regint / regvol, nor_t = a x b + b x c + c x a
+  // = (b - a) x (c - a) (render.py:208,232)
+  const double sc = (regint != 0.0f && regvol != 0.0) ? (double)regint / regvol : 0.0;
+  double u[3], q[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    u[d] = (double)P[1][d] - (double)P[0][d];
+    q[d] = (double)P[2][d] - (double)P[0][d];
+  }
+  r.an[0] = (float)(sc * (u[1] * q[2] - u[2] * q[1]));
+  r.an[1] = (float)(sc * (u[2] * q[0] - u[0] * q[2]));
+  r.an[2] = (float)(sc * (u[0] * q[1] - u[1] * q[0]));
   out[t] = r;
 }
 
 // -------------------------------------------------------------------- forward
 
+// kAF: autofluorescence variant (fourier_from_mesh_narrowband_autofluorescence,
+// render.py:228-318): each face weighs w_t + i (xi . an_t) / |xi|^2, so the
+// lane also sums (xi . an_t) F_t and adds i/|xi|^2 times it at the end; at
+// xi = 0 that term is its limit -regint (the solid's transform at 0 over
+// regvol; the reference divides by zero there).
+template <bool kAF>
 __global__ __launch_bounds__(256) void meshft_fwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, int64_t faces_per_split,
     const float* __restrict__ xi, int64_t n, float2* __restrict__ out,
-    double2* __restrict__ part) {
+    double2* __restrict__ part, const double* __restrict__ S_in, float regint) {
   __shared__ FaceRec tile[kTile];
   const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool active = m < n;
@@ -159,7 +191,7 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
   const double X = x, Y = y, Z = z;
   const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
   const int64_t f1 = (f0 + faces_per_split < nf) ? f0 + faces_per_split : nf;
-  double accr = 0.0, acci = 0.0;
+  double accr = 0.0, acci = 0.0, acnr = 0.0, acni = 0.0;
   for (int64_t base = f0; base < f1; base += kTile) {
     const int cnt = (int)((f1 - base < kTile) ? f1 - base : kTile);
     __syncthreads();
@@ -167,7 +199,7 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
     __syncthreads();
     for (int k0 = 0; k0 < cnt; k0 += kFold) {
       const int k1 = (k0 + kFold < cnt) ? k0 + kFold : cnt;
-      float sr = 0.0f, si = 0.0f;
+      float sr = 0.0f, si = 0.0f, tr = 0.0f, ti = 0.0f;
       for (int k = k0; k < k1; ++k) {
         const FaceRec& r = tile[k];
         const double rev_a = fma(r.a[2], Z, fma(r.a[1], Y, r.a[0] * X));
@@ -183,9 +215,27 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
         const float wm = r.w * m;
         sr = __builtin_fmaf(wm, F.re, sr);
         si = __builtin_fmaf(wm, F.im, si);
+        if (kAF) {
+          const float tm = __builtin_fmaf(r.an[2], z, __builtin_fmaf(r.an[1], y, r.an[0] * x)) * m;
+          tr = __builtin_fmaf(tm, F.re, tr);
+          ti = __builtin_fmaf(tm, F.im, ti);
+        }
       }
       accr += (double)sr;
       acci += (double)si;
+      if (kAF) {
+        acnr += (double)tr;
+        acni += (double)ti;
+      }
+    }
+  }
+  if (kAF && active) {
+    const double q = X * X + Y * Y + Z * Z;
+    if (q > 0.0) {
+      accr -= acni / q;
+      acci += acnr / q;
+    } else if (blockIdx.y == 0 && S_in[1] != 0.0) {
+      accr -= (double)regint;
     }
   }
   if (active) {
@@ -212,26 +262,39 @@ __global__ __launch_bounds__(256) void fwd_split_reduce_kernel(const double2* __
 
 // -------------------------------------------------------------------- adjoint
 
+// Per-face sums of the adjoint: 10, or 13 for the autofluorescence variant,
+// whose face weight is W_t = w_t + i s / |xi|^2 with s = xi . an_t:
+//   P_a, P_b, T use conj(g) W_t (the face weight inside the sum),
+//   R = sum Re(conj g F) (area term, as in the plain transform),
+//   N = sum Re(conj g i F / |xi|^2) xi (derivative w.r.t. an_t).
+template <bool kAF>
+constexpr int kFaceSums = kAF ? 13 : 10;
+
+template <bool kAF>
 __global__ __launch_bounds__(256) void meshft_bwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, const float* __restrict__ xi,
     const float* __restrict__ grad, int64_t n, int64_t freqs_per_chunk,
-    double* __restrict__ part /* [nchunk][10][nf] */) {
+    double* __restrict__ part /* [nchunk][kFaceSums][nf] */) {
+  constexpr int K = kFaceSums<kAF>;
   __shared__ FreqRec tile[kTile];
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool active = t < nf;
   double A0 = 0.0, A1 = 0.0, A2 = 0.0;
   float b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+  float fw = 0.0f, n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
   if (active) {
     const FaceRec r = faces[t];
     A0 = r.a[0]; A1 = r.a[1]; A2 = r.a[2];
     b0 = r.eb[0]; b1 = r.eb[1]; b2 = r.eb[2];
     c0 = r.ec[0]; c1 = r.ec[1]; c2 = r.ec[2];
+    fw = r.w;
+    n0 = r.an[0]; n1 = r.an[1]; n2 = r.an[2];
   }
   const int64_t m0 = (int64_t)blockIdx.y * freqs_per_chunk;
   const int64_t m1 = (m0 + freqs_per_chunk < n) ? m0 + freqs_per_chunk : n;
-  double acc[10];
+  double acc[K];
 #pragma unroll
-  for (int i = 0; i < 10; ++i) acc[i] = 0.0;
+  for (int i = 0; i < K; ++i) acc[i] = 0.0;
   for (int64_t base = m0; base < m1; base += kTile) {
     const int cnt = (int)((m1 - base < kTile) ? m1 - base : kTile);
     __syncthreads();
@@ -242,15 +305,16 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
       fr.X[0] = fr.x[0]; fr.X[1] = fr.x[1]; fr.X[2] = fr.x[2];
       fr.gr = grad[2 * m];
       fr.gi = grad[2 * m + 1];
-      fr.pad = 0.0f;
+      const double q = fr.X[0] * fr.X[0] + fr.X[1] * fr.X[1] + fr.X[2] * fr.X[2];
+      fr.inv2 = (kAF && q > 0.0) ? (float)(1.0 / q) : 0.0f;
       tile[threadIdx.x] = fr;
     }
     __syncthreads();
     for (int k0 = 0; k0 < cnt; k0 += kFold) {
       const int k1 = (k0 + kFold < cnt) ? k0 + kFold : cnt;
-      float a[10];
+      float a[K];
 #pragma unroll
-      for (int i = 0; i < 10; ++i) a[i] = 0.0f;
+      for (int i = 0; i < K; ++i) a[i] = 0.0f;
       for (int k = k0; k < k1; ++k) {
         const FreqRec& fr = tile[k];
         const double rev_a = fma(A2, fr.X[2], fma(A1, fr.X[1], A0 * fr.X[0]));
@@ -272,10 +336,22 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
         //   Re(conj g dF/dx_hi) = -(Re(z pQ) + rho) rD,
         //   Re(conj g dF/dx_mid) = gamma - (lo) - (hi).
         const cf z = cmul({fr.gr, -fr.gi}, em);
-        const float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im) * rD;
-        const float ga = __builtin_fmaf(z.re, d.im, z.im * d.re) * rD;
-        const float alo = (__builtin_fmaf(z.re, pP.re, -z.im * pP.im) + rh) * rD;
-        const float ahi = -(__builtin_fmaf(z.re, pQ.re, -z.im * pQ.im) + rh) * rD;
+        float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im) * rD;
+        cf zt = z;      // conj(g) e_mid times the face weight when it sits inside the sum
+        float rt = rh;  // Re(zt H)
+        if (kAF) {
+          // z2 = i conj(g) e_mid / |xi|^2; zt = w z + s z2 with s = xi . an
+          const cf z2 = {-z.im * fr.inv2, z.re * fr.inv2};
+          const float sn = __builtin_fmaf(n2, fr.x[2], __builtin_fmaf(n1, fr.x[1], n0 * fr.x[0]));
+          zt = {__builtin_fmaf(fw, z.re, sn * z2.re), __builtin_fmaf(fw, z.im, sn * z2.im)};
+          rt = __builtin_fmaf(zt.re, d.re, -zt.im * d.im) * rD;
+          const float nn = __builtin_fmaf(z2.re, d.re, -z2.im * d.im) * rD;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) a[10 + j] = __builtin_fmaf(nn, fr.x[j], a[10 + j]);
+        }
+        const float ga = __builtin_fmaf(zt.re, d.im, zt.im * d.re) * rD;
+        const float alo = (__builtin_fmaf(zt.re, pP.re, -zt.im * pP.im) + rt) * rD;
+        const float ahi = -(__builtin_fmaf(zt.re, pQ.re, -zt.im * pQ.im) + rt) * rD;
         const float amid = ga - alo - ahi;
         const int ra = (p < 0.0f) + (q < 0.0f);
         const int rb = (p >= 0.0f) + (q < p);
@@ -290,13 +366,13 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
         a[9] += rh;
       }
 #pragma unroll
-      for (int i = 0; i < 10; ++i) acc[i] += a[i];
+      for (int i = 0; i < K; ++i) acc[i] += a[i];
     }
   }
   if (active) {
-    double* dst = part + (int64_t)blockIdx.y * 10 * nf + t;
+    double* dst = part + (int64_t)blockIdx.y * K * nf + t;
 #pragma unroll
-    for (int i = 0; i < 10; ++i) dst[(int64_t)i * nf] = acc[i];
+    for (int i = 0; i < K; ++i) dst[(int64_t)i * nf] = acc[i];
   }
 }
 
@@ -306,21 +382,28 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
 // and wR_t = w_t R_t (for K = sum_t w_t R_t = sum_m Re(conj g_m ftmesh_m)).
 // dA/dv_j = 1/2 n_hat x (v_{j+2} - v_{j+1}); a zero-area face has no normal
 // and contributes no area gradient (the reference divides by zero, .cu:24).
-template <typename IdxT>
+// Autofluorescence variant (P_tj already carries the face weight):
+//   G_tj  = P_tj + (2 c_t / S) R_t dA_t/dv_j + (regint / regvol) (v_{j+1} - v_{j+2}) x N_t
+//   cgV_tj = d regvol / dv_j = -(v_{j+1} x v_{j+2}) / 6,  qN_t = nor_t . N_t
+// (nor_t . N_t is the derivative w.r.t. nor_t; its regvol dependence is the
+// global term -(regint / regvol^2) sum_t qN_t cgV_tj, added in the gather).
+template <typename IdxT, bool kAF>
 __global__ __launch_bounds__(256) void bwd_face_finalize_kernel(
     const float* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
     const float* __restrict__ coeff, const double* __restrict__ cA, const double* __restrict__ S_in,
-    const double* __restrict__ part, int nchunk, double* __restrict__ G, double* __restrict__ cgA,
-    double* __restrict__ wR, int* flag) {
+    const double* __restrict__ part, int nchunk, float regint, double* __restrict__ G,
+    double* __restrict__ cgA, double* __restrict__ wR, double* __restrict__ cgV,
+    double* __restrict__ qN, int* flag) {
+  constexpr int K = kFaceSums<kAF>;
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= nf) return;
-  double s[10];
+  double s[K];
 #pragma unroll
-  for (int i = 0; i < 10; ++i) s[i] = 0.0;
+  for (int i = 0; i < K; ++i) s[i] = 0.0;
   for (int c = 0; c < nchunk; ++c) {
-    const double* src = part + (int64_t)c * 10 * nf + t;
+    const double* src = part + (int64_t)c * K * nf + t;
 #pragma unroll
-    for (int i = 0; i < 10; ++i) s[i] += src[(int64_t)i * nf];
+    for (int i = 0; i < K; ++i) s[i] += src[(int64_t)i * nf];
   }
   float Pf[3][3];
   load_face_verts(V, nv, Fc, t, Pf, flag);
@@ -336,6 +419,9 @@ __global__ __launch_bounds__(256) void bwd_face_finalize_kernel(
     q[d] = P[2][d] - P[0][d];
   }
   double nr[3] = {u[1] * q[2] - u[2] * q[1], u[2] * q[0] - u[0] * q[2], u[0] * q[1] - u[1] * q[0]};
+  if (kAF) {
+    qN[t] = nr[0] * s[10] + nr[1] * s[11] + nr[2] * s[12];
+  }
   const double nn = sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
   const double inv = (nn > 0.0) ? 1.0 / nn : 0.0;
 #pragma unroll
@@ -344,6 +430,9 @@ __global__ __launch_bounds__(256) void bwd_face_finalize_kernel(
   const double ct = (double)coeff[t];
   const double w = (S > 0.0) ? 2.0 * cA[t] / S : 0.0;
   const double aw = (S > 0.0) ? 2.0 * ct / S * s[9] : 0.0;
+  const double regvol = S_in[1];
+  const double sc = (kAF && regint != 0.0f && regvol != 0.0) ? (double)regint / regvol : 0.0;
+  const double pw = kAF ? 1.0 : w;  // the autofluorescence sums carry w_t already
   double Pj[3][3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -358,9 +447,20 @@ __global__ __launch_bounds__(256) void bwd_face_finalize_kernel(
     const double e[3] = {e2[0] - e1[0], e2[1] - e1[1], e2[2] - e1[2]};
     const double gA[3] = {0.5 * (nr[1] * e[2] - nr[2] * e[1]), 0.5 * (nr[2] * e[0] - nr[0] * e[2]),
                           0.5 * (nr[0] * e[1] - nr[1] * e[0])};
+    double gN[3] = {0.0, 0.0, 0.0};
+    if (kAF) {
+      // (v_{j+1} - v_{j+2}) x N and d regvol / dv_j = -(v_{j+1} x v_{j+2}) / 6
+      const double f[3] = {-e[0], -e[1], -e[2]};
+      gN[0] = sc * (f[1] * s[12] - f[2] * s[11]);
+      gN[1] = sc * (f[2] * s[10] - f[0] * s[12]);
+      gN[2] = sc * (f[0] * s[11] - f[1] * s[10]);
+      cgV[9 * t + 3 * j + 0] = -(e1[1] * e2[2] - e1[2] * e2[1]) / 6.0;
+      cgV[9 * t + 3 * j + 1] = -(e1[2] * e2[0] - e1[0] * e2[2]) / 6.0;
+      cgV[9 * t + 3 * j + 2] = -(e1[0] * e2[1] - e1[1] * e2[0]) / 6.0;
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      G[9 * t + 3 * j + d] = w * Pj[j][d] + aw * gA[d];
+      G[9 * t + 3 * j + d] = pw * Pj[j][d] + aw * gA[d] + gN[d];
       cgA[9 * t + 3 * j + d] = ct * gA[d];
     }
   }
@@ -457,20 +557,31 @@ __global__ __launch_bounds__(256) void csr_sort_rows_kernel(const int* __restric
   }
 }
 
+// grad_v = sum over incidences (t, j) of G_tj - (K/S) cgA_tj, and for the
+// autofluorescence variant (cgV != null) - (regint Kv / regvol^2) cgV_tj
+// with Kv = sum_t qN_t.
 __global__ __launch_bounds__(256) void vertex_gather_kernel(
     const int* __restrict__ row, const int* __restrict__ ent, int64_t nv,
     const double* __restrict__ G, const double* __restrict__ cgA, const double* __restrict__ K_in,
-    const double* __restrict__ S_in, float* __restrict__ grad) {
+    const double* __restrict__ S_in, const double* __restrict__ cgV,
+    const double* __restrict__ Kv_in, float regint, float* __restrict__ grad) {
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= nv) return;
   const double S = S_in[0];
   const double kS = (S > 0.0) ? K_in[0] / S : 0.0;
+  const double regvol = S_in[1];
+  const double kv = (cgV && regvol != 0.0) ? (double)regint * Kv_in[0] / (regvol * regvol) : 0.0;
   double g0 = 0.0, g1 = 0.0, g2 = 0.0;
   for (int i = row[v]; i < row[v + 1]; ++i) {
     const int64_t e = ent[i];  // = 3 t + j, so G[3e + d] = G[t][j][d]
     g0 += G[3 * e] - kS * cgA[3 * e];
     g1 += G[3 * e + 1] - kS * cgA[3 * e + 1];
     g2 += G[3 * e + 2] - kS * cgA[3 * e + 2];
+    if (cgV) {
+      g0 -= kv * cgV[3 * e];
+      g1 -= kv * cgV[3 * e + 1];
+      g2 -= kv * cgV[3 * e + 2];
+    }
   }
   grad[3 * v] = (float)g0;
   grad[3 * v + 1] = (float)g1;
@@ -585,13 +696,16 @@ struct BwdWs {
   double* cgA;
   double* wR;
   double* K;
+  double* cgV;  // autofluorescence variant only
+  double* qN;
+  double* Kv;
   int* cnt;
   int* fillc;
   int* row;
   int* ent;
 };
 
-static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n) {
+static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n, bool af) {
   BwdWs w;
   const int64_t f1 = nf > 0 ? nf : 1, v1 = nv > 0 ? nv : 1;
   w.flag = c.take<int>(4);
@@ -599,11 +713,14 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n) {
   w.S = c.take<double>(2);
   w.recs = c.take<FaceRec>(f1);
   const BwdPlan p = bwd_plan(nf, n);
-  w.part = c.take<double>((size_t)p.nchunk * 10 * f1);
+  w.part = c.take<double>((size_t)p.nchunk * (af ? kFaceSums<true> : kFaceSums<false>) * f1);
   w.G = c.take<double>(9 * f1);
   w.cgA = c.take<double>(9 * f1);
   w.wR = c.take<double>(f1);
   w.K = c.take<double>(2);
+  w.cgV = af ? c.take<double>(9 * f1) : nullptr;
+  w.qN = af ? c.take<double>(f1) : nullptr;
+  w.Kv = af ? c.take<double>(2) : nullptr;
   w.cnt = c.take<int>(v1);
   w.fillc = c.take<int>(v1);
   w.row = c.take<int>(v1 + 1);
@@ -613,20 +730,21 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n) {
 
 template <typename IdxT>
 static int prep_faces(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, const float* coeff,
-                      int* flag, double* cA, double* S, FaceRec* recs, hipStream_t st) {
+                      float regint, int* flag, double* cA, double* S, FaceRec* recs,
+                      hipStream_t st) {
   hipLaunchKernelGGL(face_area_sum_kernel<IdxT>, dim3(1), dim3(1024), 0, st, V, nv, Fc, nf, coeff,
                      cA, S, flag);
   DMIC_LAUNCH_CHECK();
   hipLaunchKernelGGL(face_pack_kernel<IdxT>, dim3(ceil_div(nf, 256)), dim3(256), 0, st, V, nv, Fc,
-                     nf, cA, S, recs, flag);
+                     nf, cA, S, regint, recs, flag);
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
 }
 
-template <typename IdxT>
+template <typename IdxT, bool kAF>
 static int forward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, const float* coeff,
-                        const float* xi, int64_t n, float* out, void* ws, size_t ws_bytes,
-                        hipStream_t st) {
+                        float regint, const float* xi, int64_t n, float* out, void* ws,
+                        size_t ws_bytes, hipStream_t st) {
   Carver c(ws);
   FwdWs w = carve_fwd(c, nf, n);
   DMIC_REQUIRE(ws_bytes >= c.bytes(), DMIC_ERR_WORKSPACE,
@@ -640,14 +758,16 @@ static int forward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, 
     DMIC_LAUNCH_CHECK();
     return DMIC_OK;
   }
-  int rc = prep_faces<IdxT>(V, nv, Fc, nf, coeff, w.flag, w.cA, w.S, w.recs, st);
+  int rc = prep_faces<IdxT>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S, w.recs,
+                            st);
   if (rc != DMIC_OK) return rc;
   const FwdPlan p = fwd_plan(nf, n);
   dim3 grid(ceil_div(n, 256), p.nsplit);
   rc = timer_begin(0, st);
   if (rc != DMIC_OK) return rc;
-  hipLaunchKernelGGL(meshft_fwd_kernel, grid, dim3(256), 0, st, w.recs, nf, p.faces_per_split, xi,
-                     n, reinterpret_cast<float2*>(out), p.nsplit > 1 ? w.part : nullptr);
+  hipLaunchKernelGGL(meshft_fwd_kernel<kAF>, grid, dim3(256), 0, st, w.recs, nf, p.faces_per_split,
+                     xi, n, reinterpret_cast<float2*>(out), p.nsplit > 1 ? w.part : nullptr, w.S,
+                     regint);
   DMIC_LAUNCH_CHECK();
   rc = timer_end(0, st);
   if (rc != DMIC_OK) return rc;
@@ -659,12 +779,13 @@ static int forward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, 
   return DMIC_OK;
 }
 
-template <typename IdxT>
+template <typename IdxT, bool kAF>
 static int backward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf,
-                         const float* coeff, const float* grad, const float* xi, int64_t n,
-                         float* gverts, void* ws, size_t ws_bytes, hipStream_t st) {
+                         const float* coeff, float regint, const float* grad, const float* xi,
+                         int64_t n, float* gverts, void* ws, size_t ws_bytes, hipStream_t st) {
+  constexpr int K = kFaceSums<kAF>;
   Carver c(ws);
-  BwdWs w = carve_bwd(c, nv, nf, n);
+  BwdWs w = carve_bwd(c, nv, nf, n, kAF);
   DMIC_REQUIRE(ws_bytes >= c.bytes(), DMIC_ERR_WORKSPACE,
                "backward workspace too small: %zu < %zu bytes", ws_bytes, c.bytes());
   hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, w.flag, 4, 0);
@@ -676,29 +797,35 @@ static int backward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf,
     DMIC_LAUNCH_CHECK();
     return DMIC_OK;
   }
-  int rc = prep_faces<IdxT>(V, nv, Fc, nf, coeff, w.flag, w.cA, w.S, w.recs, st);
+  int rc = prep_faces<IdxT>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S, w.recs,
+                            st);
   if (rc != DMIC_OK) return rc;
   const BwdPlan p = bwd_plan(nf, n);
   if (n > 0) {
     dim3 grid(ceil_div(nf, 256), p.nchunk);
     rc = timer_begin(1, st);
     if (rc != DMIC_OK) return rc;
-    hipLaunchKernelGGL(meshft_bwd_kernel, grid, dim3(256), 0, st, w.recs, nf, xi, grad, n,
+    hipLaunchKernelGGL(meshft_bwd_kernel<kAF>, grid, dim3(256), 0, st, w.recs, nf, xi, grad, n,
                        p.freqs_per_chunk, w.part);
     DMIC_LAUNCH_CHECK();
     rc = timer_end(1, st);
     if (rc != DMIC_OK) return rc;
   } else {
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(20 * nf, 256)), dim3(256), 0, st,
-                       reinterpret_cast<int*>(w.part), 20 * nf, 0);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * K * nf, 256)), dim3(256), 0, st,
+                       reinterpret_cast<int*>(w.part), 2 * K * nf, 0);
     DMIC_LAUNCH_CHECK();
   }
   const int nchunk = (n > 0) ? p.nchunk : 1;
-  hipLaunchKernelGGL(bwd_face_finalize_kernel<IdxT>, dim3(ceil_div(nf, 256)), dim3(256), 0, st, V,
-                     nv, Fc, nf, coeff, w.cA, w.S, w.part, nchunk, w.G, w.cgA, w.wR, w.flag);
+  hipLaunchKernelGGL((bwd_face_finalize_kernel<IdxT, kAF>), dim3(ceil_div(nf, 256)), dim3(256), 0,
+                     st, V, nv, Fc, nf, coeff, w.cA, w.S, w.part, nchunk, regint, w.G, w.cgA, w.wR,
+                     w.cgV, w.qN, w.flag);
   DMIC_LAUNCH_CHECK();
   hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(1024), 0, st, w.wR, nf, w.K);
   DMIC_LAUNCH_CHECK();
+  if (kAF) {
+    hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(1024), 0, st, w.qN, nf, w.Kv);
+    DMIC_LAUNCH_CHECK();
+  }
   // CSR face -> vertex (rows sorted by incidence id 3t+j)
   hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(nv, 256)), dim3(256), 0, st, w.cnt, nv, 0);
   DMIC_LAUNCH_CHECK();
@@ -716,7 +843,7 @@ static int backward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf,
                      w.ent);
   DMIC_LAUNCH_CHECK();
   hipLaunchKernelGGL(vertex_gather_kernel, dim3(ceil_div(nv, 256)), dim3(256), 0, st, w.row, w.ent,
-                     nv, w.G, w.cgA, w.K, w.S, gverts);
+                     nv, w.G, w.cgA, w.K, w.S, w.cgV, w.Kv, regint, gverts);
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
 }
@@ -742,14 +869,26 @@ size_t dmic_meshft_forward_workspace(int64_t nv, int64_t nf, int64_t n) {
 size_t dmic_meshft_backward_workspace(int64_t nv, int64_t nf, int64_t n) {
   if (nv < 0 || nf < 0 || n < 0) return 0;
   Carver c(nullptr);
-  carve_bwd(c, nv, nf, n);
+  carve_bwd(c, nv, nf, n, false);
   return c.bytes();
 }
 
-int dmic_meshft_forward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
-                        int64_t nf, const float* faces_coeff, const float* xi_list, int64_t n,
-                        float* ftmesh_list, void* workspace, size_t workspace_bytes,
-                        hipStream_t stream) {
+size_t dmic_meshft_af_backward_workspace(int64_t nv, int64_t nf, int64_t n) {
+  if (nv < 0 || nf < 0 || n < 0) return 0;
+  Carver c(nullptr);
+  carve_bwd(c, nv, nf, n, true);
+  return c.bytes();
+}
+
+}  // extern "C"
+
+namespace {
+
+template <bool kAF>
+int forward_entry(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                  int64_t nf, const float* faces_coeff, float regint, const float* xi_list,
+                  int64_t n, float* ftmesh_list, void* workspace, size_t workspace_bytes,
+                  hipStream_t stream) {
   g_err[0] = 0;
   DMIC_REQUIRE(nv >= 0 && nf >= 0 && n >= 0, DMIC_ERR_ARG, "negative size");
   DMIC_REQUIRE(nf < (int64_t(1) << 31) / 3 && n < (int64_t(1) << 40), DMIC_ERR_ARG, "size too large");
@@ -758,18 +897,20 @@ int dmic_meshft_forward(const float* verts, int64_t nv, const void* faces, int f
                "null mesh pointer");
   DMIC_REQUIRE(workspace || workspace_bytes == 0, DMIC_ERR_ARG, "null workspace");
   if (faces_is_int64)
-    return forward_impl<int64_t>(verts, nv, static_cast<const int64_t*>(faces), nf, faces_coeff,
-                                 xi_list, n, ftmesh_list, workspace, workspace_bytes, stream);
-  return forward_impl<int32_t>(verts, nv, static_cast<const int32_t*>(faces), nf, faces_coeff,
-                               xi_list, n, ftmesh_list, workspace, workspace_bytes, stream);
+    return forward_impl<int64_t, kAF>(verts, nv, static_cast<const int64_t*>(faces), nf,
+                                      faces_coeff, regint, xi_list, n, ftmesh_list, workspace,
+                                      workspace_bytes, stream);
+  return forward_impl<int32_t, kAF>(verts, nv, static_cast<const int32_t*>(faces), nf, faces_coeff,
+                                    regint, xi_list, n, ftmesh_list, workspace, workspace_bytes,
+                                    stream);
 }
 
-int dmic_meshft_backward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
-                         int64_t nf, const float* faces_coeff, const float* grad_output_list,
-                         const float* ftmesh_list, const float* xi_list, int64_t n,
-                         float* grad_verts, void* workspace, size_t workspace_bytes,
-                         hipStream_t stream) {
-  (void)ftmesh_list;
+template <bool kAF>
+int backward_entry(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                   int64_t nf, const float* faces_coeff, float regint,
+                   const float* grad_output_list, const float* xi_list, int64_t n,
+                   float* grad_verts, void* workspace, size_t workspace_bytes,
+                   hipStream_t stream) {
   g_err[0] = 0;
   DMIC_REQUIRE(nv >= 0 && nf >= 0 && n >= 0, DMIC_ERR_ARG, "negative size");
   DMIC_REQUIRE(nf < (int64_t(1) << 31) / 3 && nv < (int64_t(1) << 31) && n < (int64_t(1) << 40),
@@ -780,12 +921,53 @@ int dmic_meshft_backward(const float* verts, int64_t nv, const void* faces, int 
                "null xi_list/grad_output_list");
   DMIC_REQUIRE(workspace || workspace_bytes == 0, DMIC_ERR_ARG, "null workspace");
   if (faces_is_int64)
-    return backward_impl<int64_t>(verts, nv, static_cast<const int64_t*>(faces), nf, faces_coeff,
-                                  grad_output_list, xi_list, n, grad_verts, workspace,
-                                  workspace_bytes, stream);
-  return backward_impl<int32_t>(verts, nv, static_cast<const int32_t*>(faces), nf, faces_coeff,
-                                grad_output_list, xi_list, n, grad_verts, workspace,
-                                workspace_bytes, stream);
+    return backward_impl<int64_t, kAF>(verts, nv, static_cast<const int64_t*>(faces), nf,
+                                       faces_coeff, regint, grad_output_list, xi_list, n,
+                                       grad_verts, workspace, workspace_bytes, stream);
+  return backward_impl<int32_t, kAF>(verts, nv, static_cast<const int32_t*>(faces), nf,
+                                     faces_coeff, regint, grad_output_list, xi_list, n,
+                                     grad_verts, workspace, workspace_bytes, stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dmic_meshft_forward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                        int64_t nf, const float* faces_coeff, const float* xi_list, int64_t n,
+                        float* ftmesh_list, void* workspace, size_t workspace_bytes,
+                        hipStream_t stream) {
+  return forward_entry<false>(verts, nv, faces, faces_is_int64, nf, faces_coeff, 0.0f, xi_list, n,
+                              ftmesh_list, workspace, workspace_bytes, stream);
+}
+
+int dmic_meshft_backward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                         int64_t nf, const float* faces_coeff, const float* grad_output_list,
+                         const float* ftmesh_list, const float* xi_list, int64_t n,
+                         float* grad_verts, void* workspace, size_t workspace_bytes,
+                         hipStream_t stream) {
+  (void)ftmesh_list;
+  return backward_entry<false>(verts, nv, faces, faces_is_int64, nf, faces_coeff, 0.0f,
+                               grad_output_list, xi_list, n, grad_verts, workspace,
+                               workspace_bytes, stream);
+}
+
+int dmic_meshft_af_forward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                           int64_t nf, const float* faces_coeff, float regint,
+                           const float* xi_list, int64_t n, float* ftmesh_list, void* workspace,
+                           size_t workspace_bytes, hipStream_t stream) {
+  return forward_entry<true>(verts, nv, faces, faces_is_int64, nf, faces_coeff, regint, xi_list, n,
+                             ftmesh_list, workspace, workspace_bytes, stream);
+}
+
+int dmic_meshft_af_backward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                            int64_t nf, const float* faces_coeff, float regint,
+                            const float* grad_output_list, const float* xi_list, int64_t n,
+                            float* grad_verts, void* workspace, size_t workspace_bytes,
+                            hipStream_t stream) {
+  return backward_entry<true>(verts, nv, faces, faces_is_int64, nf, faces_coeff, regint,
+                              grad_output_list, xi_list, n, grad_verts, workspace,
+                              workspace_bytes, stream);
 }
 
 int dmic_profile_enable(int enable) {

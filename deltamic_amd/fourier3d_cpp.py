@@ -69,11 +69,14 @@ def validate_faces(mesh_faces: torch.Tensor, nv: int) -> None:
 
 def forward_fourier3d_cpp(mesh_vertices: torch.Tensor, mesh_faces: torch.Tensor,
                           faces_coeff: torch.Tensor, xi_list: torch.Tensor,
-                          out: torch.Tensor | None = None) -> torch.Tensor:
+                          out: torch.Tensor | None = None,
+                          regint: float | None = None) -> torch.Tensor:
     """ftmesh_list[n] complex64 (fourier3d_cuda.cpp:27-42, .cu:421-462).
 
     ``out`` (optional, complex64, at least n entries) receives the values in
-    its first n entries."""
+    its first n entries.  ``regint`` (not in the reference's module) selects
+    the autofluorescence variant of render.py:228-318: face weights
+    w_t + i (xi . nor2_t) / |xi|^2 with nor2 = nor regint / regvol."""
     _check_mesh(mesh_vertices, mesh_faces, faces_coeff)
     _check_input(xi_list, "xi_list")
     if xi_list.dtype != torch.float32 or xi_list.dim() != 2 or xi_list.shape[1] != 3:
@@ -89,18 +92,27 @@ def forward_fourier3d_cpp(mesh_vertices: torch.Tensor, mesh_faces: torch.Tensor,
     L = _lib.lib()
     nbytes = L.dmic_meshft_forward_workspace(nv, nf, n)
     ws = _lib.workspace(nbytes, dev)
-    rc = L.dmic_meshft_forward(
-        mesh_vertices.data_ptr(), nv, mesh_faces.data_ptr(), int(mesh_faces.dtype == torch.int64), nf,
-        faces_coeff.data_ptr(), xi_list.data_ptr(), n, out.data_ptr(), ws.data_ptr(), ws.numel(),
-        _lib.stream_of(out))
-    _lib.check(rc, "dmic_meshft_forward")
+    is64 = int(mesh_faces.dtype == torch.int64)
+    if regint is None:
+        rc = L.dmic_meshft_forward(
+            mesh_vertices.data_ptr(), nv, mesh_faces.data_ptr(), is64, nf, faces_coeff.data_ptr(),
+            xi_list.data_ptr(), n, out.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_of(out))
+        _lib.check(rc, "dmic_meshft_forward")
+    else:
+        rc = L.dmic_meshft_af_forward(
+            mesh_vertices.data_ptr(), nv, mesh_faces.data_ptr(), is64, nf, faces_coeff.data_ptr(),
+            float(regint), xi_list.data_ptr(), n, out.data_ptr(), ws.data_ptr(), ws.numel(),
+            _lib.stream_of(out))
+        _lib.check(rc, "dmic_meshft_af_forward")
     return out
 
 
 def backward_fourier3d_cpp(mesh_vertices: torch.Tensor, mesh_faces: torch.Tensor,
                            faces_coeff: torch.Tensor, grad_output_list: torch.Tensor,
-                           ftmesh_list: torch.Tensor, xi_list: torch.Tensor) -> torch.Tensor:
-    """gradVerts[nv, 3] float32 (fourier3d_cuda.cpp:44-65, .cu:466-539)."""
+                           ftmesh_list: torch.Tensor, xi_list: torch.Tensor,
+                           regint: float | None = None) -> torch.Tensor:
+    """gradVerts[nv, 3] float32 (fourier3d_cuda.cpp:44-65, .cu:466-539); with
+    ``regint``, the gradient of the autofluorescence variant."""
     _check_mesh(mesh_vertices, mesh_faces, faces_coeff)
     _check_input(grad_output_list, "grad_output_list")
     _check_input(ftmesh_list, "ftmesh_list")
@@ -115,11 +127,21 @@ def backward_fourier3d_cpp(mesh_vertices: torch.Tensor, mesh_faces: torch.Tensor
     dev = mesh_vertices.device
     grad = torch.empty((nv, 3), dtype=torch.float32, device=dev)
     L = _lib.lib()
-    nbytes = L.dmic_meshft_backward_workspace(nv, nf, n)
-    ws = _lib.workspace(nbytes, dev)
-    rc = L.dmic_meshft_backward(
-        mesh_vertices.data_ptr(), nv, mesh_faces.data_ptr(), int(mesh_faces.dtype == torch.int64), nf,
-        faces_coeff.data_ptr(), grad_output_list.data_ptr(), ftmesh_list.data_ptr(),
-        xi_list.data_ptr(), n, grad.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_of(grad))
-    _lib.check(rc, "dmic_meshft_backward")
+    is64 = int(mesh_faces.dtype == torch.int64)
+    if regint is None:
+        nbytes = L.dmic_meshft_backward_workspace(nv, nf, n)
+        ws = _lib.workspace(nbytes, dev)
+        rc = L.dmic_meshft_backward(
+            mesh_vertices.data_ptr(), nv, mesh_faces.data_ptr(), is64, nf, faces_coeff.data_ptr(),
+            grad_output_list.data_ptr(), ftmesh_list.data_ptr(), xi_list.data_ptr(), n,
+            grad.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_of(grad))
+        _lib.check(rc, "dmic_meshft_backward")
+    else:
+        nbytes = L.dmic_meshft_af_backward_workspace(nv, nf, n)
+        ws = _lib.workspace(nbytes, dev)
+        rc = L.dmic_meshft_af_backward(
+            mesh_vertices.data_ptr(), nv, mesh_faces.data_ptr(), is64, nf, faces_coeff.data_ptr(),
+            float(regint), grad_output_list.data_ptr(), xi_list.data_ptr(), n, grad.data_ptr(),
+            ws.data_ptr(), ws.numel(), _lib.stream_of(grad))
+        _lib.check(rc, "dmic_meshft_af_backward")
     return grad

@@ -133,28 +133,30 @@ def gather_band(box: torch.Tensor, band: Band) -> torch.Tensor:
     return vals
 
 
-def band_forward(V, Fc, C, band: Band) -> torch.Tensor:
-    """ftmesh_list [n] complex64 on the band (half band + mirror when symmetric)."""
+def band_forward(V, Fc, C, band: Band, regint=None) -> torch.Tensor:
+    """ftmesh_list [n] complex64 on the band (half band + mirror when symmetric;
+    both variants are Hermitian: real face weights and xi . nor2 odd in xi)."""
     if not (HERMITIAN_HALVING and band.symmetric and band.n > 1):
-        return forward_fourier3d_cpp(V, Fc, C, band.xi_list)
+        return forward_fourier3d_cpp(V, Fc, C, band.xi_list, regint=regint)
     out = torch.empty((band.n,), dtype=torch.complex64, device=V.device)
-    forward_fourier3d_cpp(V, Fc, C, band.xi_list[:band.half], out=out)
+    forward_fourier3d_cpp(V, Fc, C, band.xi_list[:band.half], out=out, regint=regint)
     L = _lib.lib()
     _lib.check(L.dmic_band_mirror_fill(out.data_ptr(), band.n, _lib.stream_of(out)),
                "dmic_band_mirror_fill")
     return out
 
 
-def band_backward(V, Fc, C, grad_list, ftmesh_list, band: Band) -> torch.Tensor:
+def band_backward(V, Fc, C, grad_list, ftmesh_list, band: Band, regint=None) -> torch.Tensor:
     """gradVerts [nv, 3] from the band upstream gradient (folded to the half band when symmetric)."""
     if not (HERMITIAN_HALVING and band.symmetric and band.n > 1):
-        return backward_fourier3d_cpp(V, Fc, C, grad_list, ftmesh_list, band.xi_list)
+        return backward_fourier3d_cpp(V, Fc, C, grad_list, ftmesh_list, band.xi_list,
+                                      regint=regint)
     g_half = torch.empty((band.half,), dtype=torch.complex64, device=V.device)
     L = _lib.lib()
     _lib.check(L.dmic_band_mirror_fold(grad_list.data_ptr(), band.n, g_half.data_ptr(),
                                        _lib.stream_of(g_half)), "dmic_band_mirror_fold")
     return backward_fourier3d_cpp(V, Fc, C, g_half, ftmesh_list[:band.half],
-                                  band.xi_list[:band.half])
+                                  band.xi_list[:band.half], regint=regint)
 
 
 class MeshFTHIP(Function):
@@ -182,6 +184,32 @@ class MeshFTHIP(Function):
         band = ctx.band
         grad_list = gather_band(grad_output, band)
         gradVerts = band_backward(V, Fc, C, grad_list, ftmesh_list, band)
+        return gradVerts, None, None, None, None, None, None, None
+
+
+class MeshFTAutofluorescenceHIP(Function):
+    """Autofluorescence variant (render.py:228-318) as an autograd Function:
+    forward(Verts, Faces, regint, xi0, xi1, xi2, Filter, thresh) -> box."""
+
+    @staticmethod
+    def forward(ctx, Verts, Faces, regint, xi0, xi1, xi2, Filter, narrowband_thresh):
+        if Verts.dtype != torch.float32:
+            raise TypeError("Mesh FT (HIP) is implemented for float32 only, got " + str(Verts.dtype))
+        band = band_for(Filter, narrowband_thresh, xi0, xi1, xi2)
+        V = Verts.detach().contiguous()
+        Fc = Faces.contiguous()
+        C = torch.ones(Fc.shape[0], dtype=torch.float32, device=V.device)
+        r = float(regint)
+        ftmesh_list = band_forward(V, Fc, C, band, regint=r)
+        ctx.save_for_backward(V, Fc, C, ftmesh_list)
+        ctx.band, ctx.regint = band, r
+        return scatter_band(ftmesh_list, band)
+
+    @staticmethod
+    def backward(ctx, grad_output):
+        V, Fc, C, ftmesh_list = ctx.saved_tensors
+        grad_list = gather_band(grad_output, ctx.band)
+        gradVerts = band_backward(V, Fc, C, grad_list, ftmesh_list, ctx.band, regint=ctx.regint)
         return gradVerts, None, None, None, None, None, None, None
 
 

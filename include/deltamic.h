@@ -79,6 +79,37 @@ int dmic_meshft_backward(const float* verts, int64_t nv, const void* faces, int 
                          hipStream_t stream);
 
 /*
+ * Autofluorescence variant of the forward: every face weighs
+ *   W_t(xi) = w_t + i (xi . nor2_t) / |xi|^2,
+ *   nor2_t = regint / regvol * ((b - a) x (c - a)),  regvol = -sum_t a . (b x c) / 6,
+ * so ftmesh(xi) = sum_t W_t F_t(xi) is the surface signal plus a uniform
+ * interior signal of strength regint (divergence theorem).  At xi = 0 the
+ * interior term is its limit -regint (the reference divides by zero there).
+ * Same arguments as dmic_meshft_forward plus regint; the workspace is
+ * dmic_meshft_forward_workspace's.  Replaces the dense torch function
+ * fourier_from_mesh_narrowband_autofluorescence (deltamic/render.py:228-318;
+ * it has no Faces_coeff: pass ones for its weights 2 A_t / sum A).
+ */
+int dmic_meshft_af_forward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                           int64_t nf, const float* faces_coeff, float regint,
+                           const float* xi_list, int64_t n, float* ftmesh_list, void* workspace,
+                           size_t workspace_bytes, hipStream_t stream);
+
+/* Workspace bytes dmic_meshft_af_backward needs for these sizes. */
+size_t dmic_meshft_af_backward_workspace(int64_t nv, int64_t nf, int64_t n);
+
+/*
+ * grad_verts = d/dV sum_m Re(conj(g_m) ftmesh_m) of the autofluorescence
+ * variant, including the dependence of nor2_t and regvol on the vertices
+ * (the reference differentiates its dense expression with autograd).
+ */
+int dmic_meshft_af_backward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
+                            int64_t nf, const float* faces_coeff, float regint,
+                            const float* grad_output_list, const float* xi_list, int64_t n,
+                            float* grad_verts, void* workspace, size_t workspace_bytes,
+                            hipStream_t stream);
+
+/*
  * Reads (synchronously, on `stream`) the out-of-range face index flag that
  * the last dmic_meshft_forward / dmic_meshft_backward call on `workspace`
  * set (each call clears it first).  Returns DMIC_OK or DMIC_ERR_INDEX.  No

@@ -465,6 +465,61 @@ int orc_acc_forward(const double* V, int64_t nv, const int64_t* Fc, int64_t nf, 
   return 0;
 }
 
+// Accurate autofluorescence variant (fourier_from_mesh_narrowband_autofluorescence,
+// deltamic/render.py:228-318): face weight w_t + i (xi . nor2_t) / |xi|^2,
+// nor2_t = nor_t regint / regvol, nor_t = a x b + b x c + c x a,
+// regvol = -sum_t a . (b x c) / 6, w_t = 2 c_t A_t / S (the reference's
+// coefficient-free version is coeff = 1).  At xi = 0 the interior term is its
+// limit regint * (sum_t a . (b x c) / 6) / regvol = -regint.  Its adjoint is
+// checked against central differences of this function.
+int orc_acc_forward_af(const double* V, int64_t nv, const int64_t* Fc, int64_t nf,
+                       const double* coeff, double regint, const double* xi, int64_t n,
+                       double* out, int nthreads) {
+  (void)nv;
+  Geo g;
+  geometry(V, Fc, nf, coeff, g);
+  std::vector<double> nor(3 * nf);
+  long double vol = 0;
+  for (int64_t t = 0; t < nf; ++t) {
+    const double* a = V + 3 * Fc[3 * t];
+    const double* b = V + 3 * Fc[3 * t + 1];
+    const double* c = V + 3 * Fc[3 * t + 2];
+    const double ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+    const double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+    const double ca[3] = {c[1] * a[2] - c[2] * a[1], c[2] * a[0] - c[0] * a[2], c[0] * a[1] - c[1] * a[0]};
+    for (int d = 0; d < 3; ++d) nor[3 * t + d] = ab[d] + bc[d] + ca[d];
+    vol += (long double)a[0] * bc[0] + (long double)a[1] * bc[1] + (long double)a[2] * bc[2];
+  }
+  const double regvol = (double)(-vol / 6);
+  const double sc = (regvol != 0.0) ? regint / regvol : 0.0;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads)
+  for (int64_t m = 0; m < n; ++m) {
+    const double* x = xi + 3 * m;
+    const long double q = (long double)x[0] * x[0] + (long double)x[1] * x[1] + (long double)x[2] * x[2];
+    long double sr = 0, si = 0, tr = 0, ti = 0;
+    for (int64_t t = 0; t < nf; ++t) {
+      cd F;
+      face_accurate(phase(V + 3 * Fc[3 * t], x), phase(V + 3 * Fc[3 * t + 1], x),
+                    phase(V + 3 * Fc[3 * t + 2], x), &F, nullptr);
+      sr += g.w[t] * F.real();
+      si += g.w[t] * F.imag();
+      const long double s = sc * ((long double)x[0] * nor[3 * t] + (long double)x[1] * nor[3 * t + 1] +
+                                  (long double)x[2] * nor[3 * t + 2]);
+      tr += s * F.real();
+      ti += s * F.imag();
+    }
+    if (q > 0) {
+      sr -= ti / q;  // i (tr + i ti) / q
+      si += tr / q;
+    } else if (regvol != 0.0) {
+      sr -= regint;
+    }
+    out[2 * m] = (double)sr;
+    out[2 * m + 1] = (double)si;
+  }
+  return 0;
+}
+
 // Accurate adjoint: grad_verts = d/dV sum_m Re(conj(g_m) ftmesh_m), the
 // gradient convention of Grad_Mesh_NUFT_kernel (.cu:392-394).  Organised per
 // face: P_tj = sum_m Re(conj g dF_j) xi_m, R_t = sum_m Re(conj g F), then

@@ -46,11 +46,14 @@ def lib():
         L.orc_face_ref_f32.argtypes = [ctypes.c_float] * 3 + [_f]
         L.orc_acc_forward.argtypes = [_d, _i64, _l, _i64, _d, _d, _i64, _d, _int]
         L.orc_acc_backward.argtypes = [_d, _i64, _l, _i64, _d, _d, _i64, _d, _d, _d, _int]
+        L.orc_acc_forward_af.argtypes = [_d, _i64, _l, _i64, _d, ctypes.c_double, _d, _i64, _d,
+                                         _int]
         L.orc_ref_forward_f32.argtypes = [_f, _i64, _l, _i64, _f, _f, _i64, _f, _int]
         L.orc_ref_forward_f64.argtypes = [_d, _i64, _l, _i64, _d, _d, _i64, _d, _int]
         L.orc_ref_backward_f32.argtypes = [_f, _i64, _l, _i64, _f, _f, _i64, _f, _f, _f, _int]
         L.orc_ref_backward_f64.argtypes = [_d, _i64, _l, _i64, _d, _d, _i64, _d, _d, _d, _int]
-        for n in ("orc_acc_forward", "orc_acc_backward", "orc_ref_forward_f32", "orc_ref_forward_f64",
+        for n in ("orc_acc_forward", "orc_acc_backward", "orc_acc_forward_af",
+                  "orc_ref_forward_f32", "orc_ref_forward_f64",
                   "orc_ref_backward_f32", "orc_ref_backward_f64"):
             getattr(L, n).restype = _int
         _lib = L
@@ -108,6 +111,19 @@ def acc_forward(verts, faces, coeff, xi, threads=None, fp32_inputs=True):
     out = np.zeros((len(X), 2))
     rc = lib().orc_acc_forward(_p(V, _d), len(V), _p(F, _l), len(F), _p(C, _d), _p(X, _d), len(X),
                                _p(out, _d), _threads(threads))
+    assert rc == 0
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def acc_forward_af(verts, faces, coeff, regint, xi, threads=None, fp32_inputs=True):
+    """Accurate autofluorescence-variant ftmesh_list (complex128 [n]); see
+    orc_acc_forward_af."""
+    V, C, X = _inputs(verts, coeff, xi, fp32_inputs)
+    F = _c(faces, np.int64)
+    out = np.zeros((len(X), 2))
+    rc = lib().orc_acc_forward_af(_p(V, _d), len(V), _p(F, _l), len(F), _p(C, _d),
+                                  ctypes.c_double(float(regint)), _p(X, _d), len(X), _p(out, _d),
+                                  _threads(threads))
     assert rc == 0
     return out[:, 0] + 1j * out[:, 1]
 

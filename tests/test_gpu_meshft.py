@@ -351,3 +351,95 @@ def test_full_size_cfg2_properties(device):
     assert torch.equal(fs.view(torch.float32), f.view(torch.float32))
     assert lin <= 1e-6
     assert torch.equal(f2.view(torch.float32), f.view(torch.float32)) and torch.equal(a2, a)
+
+
+def test_autofluorescence_forward_config1(device):
+    """Autofluorescence variant (render.py:228-318) on config 1's full band
+    (raw vertices, unit weights, regint 0.5) against the oracle, through the
+    C-ABI with and without Hermitian halving."""
+    from deltamic_amd.fourier3d_cpp import forward_fourier3d_cpp
+    from deltamic_amd.hip_class import band_forward
+    w = syn.config1()
+    bs, axes, otf, band = _band(w, device)
+    V = torch.tensor(w.verts, device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.ones(len(w.faces), device=device)
+    ft = forward_fourier3d_cpp(V, Fc, C, band.xi_list, regint=0.5)
+    xi = band.xi_list.cpu().numpy()
+    ref = O.acc_forward_af(w.verts, w.faces, np.ones(len(w.faces)), 0.5, xi, threads=threads())
+    err = O.rel_l2(ft.cpu().numpy(), ref)
+    assert band.symmetric
+    halved = band_forward(V, Fc, C, band, regint=0.5)
+    herr = O.rel_l2(halved.cpu().numpy(), ft.cpu().numpy())
+    print(f"autofluorescence cfg1 forward rel-L2 vs FP64 oracle: {err:.3e} (n={len(xi)}); "
+          f"halved vs full {herr:.1e}")
+    assert err <= FWD_TOL
+    assert herr <= 2e-6
+
+
+def test_autofluorescence_adjoint_vs_oracle_differences(device):
+    """Gradient of sum Re(conj g ftmesh) for the autofluorescence variant
+    (including the dependence of the normals and of the enclosed volume on the
+    vertices) against 4th-order central differences of the FP64 oracle, on a
+    jittered 42-vertex icosphere and 1024 frequencies."""
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    u, f = syn.icosphere(1)
+    g = syn.rng(6101)
+    v = (0.5 * u + g.normal(0, 0.01, u.shape) + 0.3).astype(np.float32)
+    xi = g.normal(0, 6.0, (1024, 3)).astype(np.float32)
+    gr = _grad(len(xi), 6102)
+    regint = 0.6
+    V = torch.tensor(v, device=device)
+    Fc = torch.tensor(f, device=device)
+    C = torch.ones(len(f), device=device)
+    X = torch.tensor(xi, device=device)
+    ft = forward_fourier3d_cpp(V, Fc, C, X, regint=regint)
+    gv = backward_fourier3d_cpp(V, Fc, C, torch.tensor(gr, device=device), ft, X,
+                                regint=regint).cpu().numpy().astype(np.float64)
+    v64 = v.astype(np.float64)
+    ones = np.ones(len(f))
+
+    def L(vv):
+        a = O.acc_forward_af(vv, f, ones, regint, xi, threads=threads(), fp32_inputs=False)
+        return float(np.sum(gr.real * a.real + gr.imag * a.imag))
+
+    h = 1e-4
+    fd = np.zeros_like(v64)
+    for i in range(len(v64)):
+        for d in range(3):
+            vals = []
+            for k in (-2, -1, 1, 2):
+                vv = v64.copy()
+                vv[i, d] += k * h
+                vals.append(L(vv))
+            fd[i, d] = (vals[0] - 8 * vals[1] + 8 * vals[2] - vals[3]) / (12 * h)
+    err = np.linalg.norm(gv - fd) / np.linalg.norm(fd)
+    print(f"autofluorescence adjoint rel-L2 vs oracle differences: {err:.3e}")
+    assert err <= BWD_TOL
+
+
+def test_autofluorescence_module_function(device):
+    """fourier_from_mesh_narrowband_autofluorescence: band values equal the
+    C-ABI call, zeros outside the band, and autograd reaches the vertices with
+    the adjoint of the C-ABI."""
+    from deltamic_amd import fourier_from_mesh_narrowband_autofluorescence
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    w = syn.config1()
+    bs, axes, otf, band = _band(w, device)
+    V = torch.tensor(w.verts, device=device, requires_grad=True)
+    Fc = torch.tensor(w.faces, device=device)
+    box = fourier_from_mesh_narrowband_autofluorescence(otf, w.thresh, 0.5, *axes, V, Fc)
+    assert box.shape == tuple(w.box_shape) and box.dtype == torch.complex64
+    flat = box.detach().reshape(-1)
+    vals = flat[band.idx.long()]
+    C = torch.ones(len(w.faces), device=device)
+    direct = forward_fourier3d_cpp(V.detach(), Fc, C, band.xi_list, regint=0.5)
+    assert O.rel_l2(vals.cpu().numpy(), direct.cpu().numpy()) <= 2e-6
+    assert int((flat != 0).sum()) <= band.n
+    up = torch.tensor(_grad(int(np.prod(w.box_shape)), 6103).reshape(w.box_shape), device=device)
+    (box.real * up.real + box.imag * up.imag).sum().backward()
+    g_list = up.reshape(-1)[band.idx.long()].contiguous()
+    ref = backward_fourier3d_cpp(V.detach(), Fc, C, g_list, direct, band.xi_list, regint=0.5)
+    err = O.rel_l2(V.grad.cpu().numpy(), ref.cpu().numpy())
+    print(f"autofluorescence module gradient vs direct adjoint: {err:.2e}")
+    assert err <= 1e-5

@@ -155,6 +155,30 @@ def test_cube_surface_closed_form():
     assert np.max(np.abs(ref - want)) <= 1e-6
 
 
+def _cube_solid_ft(xi):
+    """int over [0,1]^3 of exp(-i xi.x) dx: a product of 1-D integrals."""
+    def I(k):
+        return 1.0 if k == 0 else np.exp(-0.5j * k) * np.sin(0.5 * k) / (0.5 * k)
+    return np.array([I(a) * I(b) * I(c) for a, b, c in xi])
+
+
+def test_autofluorescence_cube_closed_form():
+    """Autofluorescence variant (render.py:228-318) on the unit cube: the
+    interior term is -regint times the solid cube's transform over its
+    volume (divergence theorem), for either orientation of the surface, and
+    its value at xi = 0 is the limit -regint."""
+    v, f = _cube()
+    g = np.random.default_rng(6)
+    xi = np.concatenate([g.normal(0, 6, (150, 3)),
+                         np.array([[3.0, 0, 0], [0, 0, 7.5], [1e-7, 2e-7, -1e-7], [40.0, -30.0, 0.5]])])
+    for faces in (f, f[:, ::-1].copy()):
+        surf = O.acc_forward(v, faces, np.ones(len(faces)), xi, threads=2, fp32_inputs=False)
+        af = O.acc_forward_af(v, faces, np.ones(len(faces)), 0.7, xi, threads=2, fp32_inputs=False)
+        assert np.max(np.abs((af - surf) - (-0.7 * _cube_solid_ft(xi)))) <= 1e-9
+        at0 = O.acc_forward_af(v, faces, np.ones(len(faces)), 0.7, np.zeros((1, 3)), fp32_inputs=False)
+        assert abs(at0[0] - (1.0 - 0.7)) <= 1e-12
+
+
 def _small_mesh(seed=11, sub=2):
     u, f = syn.icosphere(sub)
     g = syn.rng(seed)
