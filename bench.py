@@ -124,6 +124,9 @@ def main() -> int:
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--module-steps", type=int, default=3,
                     help="extra end-to-end module steps (forward, render, backward) to time")
+    ap.add_argument("--af-steps", type=int, default=3,
+                    help="also time the autofluorescence variant (forward + adjoint) on the "
+                         "same mesh and band (reported under 'autofluorescence')")
     ap.add_argument("--psf-steps", type=int, default=5,
                     help="also time the learnable-PSF kernels (forward and sigma gradient) on "
                          "the box (HBM-bound; reported separately under 'psf')")
@@ -281,6 +284,39 @@ def main() -> int:
         module_ms = a.elapsed_time(b) / args.module_steps
         log(f"module step (fwd+render+normalize+bwd): {module_ms:.3f} ms")
 
+    af = None
+    if args.af_steps > 0:
+        wsb_af = _lib.workspace(L.dmic_meshft_af_backward_workspace(nv, nf, ne), dev)
+
+        def step_af():
+            rc = L.dmic_meshft_af_forward(V.data_ptr(), nv, Fc.data_ptr(), 1, nf, C.data_ptr(), 0.5,
+                                          band.xi_list.data_ptr(), ne, out.data_ptr(),
+                                          wsf.data_ptr(), wsf.numel(), st)
+            _lib.check(rc, "af forward")
+            if halve:
+                _lib.check(L.dmic_band_mirror_fill(out.data_ptr(), n, st), "mirror_fill")
+                _lib.check(L.dmic_band_mirror_fold(grad.data_ptr(), n, g_half.data_ptr(), st),
+                           "mirror_fold")
+            gsrc = g_half if halve else grad
+            rc = L.dmic_meshft_af_backward(V.data_ptr(), nv, Fc.data_ptr(), 1, nf, C.data_ptr(), 0.5,
+                                           gsrc.data_ptr(), band.xi_list.data_ptr(), ne,
+                                           gv.data_ptr(), wsb_af.data_ptr(), wsb_af.numel(), st)
+            _lib.check(rc, "af backward")
+
+        step_af()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.af_steps):
+            step_af()
+        b.record()
+        torch.cuda.synchronize()
+        ams = a.elapsed_time(b) / args.af_steps
+        af = {"value": round(pairs / (ams * 1e-3) / 1e9, 3), "ms_per_step": round(ams, 4),
+              "steps": args.af_steps, "regint": 0.5}
+        log(f"autofluorescence variant: {ams:.3f} ms/step, {af['value']} {UNIT}")
+
     psf = None
     if args.psf_steps > 0:
         psf = psf_leg(L, dev, w, args.psf_steps)
@@ -346,6 +382,7 @@ def main() -> int:
             "module_step_ms": None if module_ms is None else round(module_ms, 3),
         },
         "full_band_evaluation": full_band,
+        "autofluorescence": af,
         "psf": psf,
         "cpu_baseline": cpu,
     }
