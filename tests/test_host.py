@@ -250,3 +250,28 @@ def test_adam_direction_matches_torch_step():
     st, sq = opt.state[p], opt_q.state[q]
     assert int(st["step"]) == int(sq["step"]) == 4
     assert torch.allclose(st["exp_avg"], sq["exp_avg"]) and torch.allclose(st["exp_avg_sq"], sq["exp_avg_sq"])
+
+
+def test_c_abi_rejects_bad_arguments_before_any_launch():
+    """Argument errors come back as codes with a message, with nothing
+    launched (these calls never reach the GPU)."""
+    from deltamic_amd import _lib
+    L = _lib.lib()
+    null = None
+    # negative sizes
+    rc = L.dmic_meshft_forward(null, -1, null, 1, 0, null, null, 0, null, null, 0, null)
+    assert rc == -1 and b"negative" in L.dmic_last_error()
+    rc = L.dmic_meshft_af_forward(null, 3, null, 1, 1, null, 0.5, null, 4, null, null, 0, null)
+    assert rc == -1 and b"null" in L.dmic_last_error()
+    rc = L.dmic_meshft_af_backward(null, 3, null, 1, 1, null, 0.5, null, null, 4, null, null, 0,
+                                   null)
+    assert rc == -1
+    rc = L.dmic_psf_gaussian(null, null, null, null, -2, 3, 4, null, null)
+    assert rc == -1 and b"negative" in L.dmic_last_error()
+    rc = L.dmic_psf_aniso_grad(null, null, null, null, null, null, 2, 2, 2, null, null, 0, null)
+    assert rc == -1
+    # workspaces: the autofluorescence adjoint keeps three more sums per face
+    assert L.dmic_meshft_af_backward_workspace(642, 1280, 22000) > \
+        L.dmic_meshft_backward_workspace(642, 1280, 22000)
+    assert L.dmic_psf_grad_workspace(200, 200, 200) >= (200 * 200 // 16) * 6 * 8
+    assert L.dmic_psf_grad_workspace(-1, 2, 2) == 0
