@@ -1,4 +1,4 @@
-// Per-vertex proposal MLP (12 -> 128 -> 128 -> 128 -> 3, ReLU) for gfx950:
+// Per-vertex proposal MLP (15 -> 128 -> 128 -> 128 -> 3, ReLU) for gfx950:
 // inference, backpropagation and Adam, with bf16 weights and FP32
 // accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16).
 //
@@ -112,8 +112,8 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ 
 }
 
 // ------------------------------------------------------------------ MLP layout
-constexpr int64_t kIn = 12, kHid = 128, kOut = 3;
-// weights, row-major [in][out]: W1 [12][128], W2 [128][128], W3 [128][128], W4 [128][3]
+constexpr int64_t kIn = 15, kHid = 128, kOut = 3;
+// weights, row-major [in][out]: W1 [15][128], W2 [128][128], W3 [128][128], W4 [128][3]
 constexpr int64_t kW1 = 0, kW2 = kW1 + kIn * kHid, kW3 = kW2 + kHid * kHid,
                   kW4 = kW3 + kHid * kHid, kNW = kW4 + kHid * kOut;
 // biases: b1 [128], b2 [128], b3 [128], b4 [3]
@@ -173,7 +173,7 @@ int dmic_gemm_bf16(const void* A, int a_is_bf16, int64_t sa0, int64_t sa1, const
 int64_t dmic_mlp_weight_count(void) { return kNW; }
 int64_t dmic_mlp_bias_count(void) { return kNB; }
 
-// y[rows][3] = MLP(x[rows][12]); acts (optional) receives H1, H2, H3 as
+// y[rows][3] = MLP(x[rows][15]); acts (optional) receives H1, H2, H3 as
 // [3][rows][128] fp32 for dmic_mlp_backward.  w_bf16: packed weights (bf16).
 int dmic_mlp_forward(const uint16_t* w_bf16, const float* bias, const float* x, int64_t rows,
                      float* y, float* acts, float* scratch, hipStream_t stream) {

@@ -52,7 +52,7 @@ def collect(args, device):
     # calibrate the gradient feature scale to unit RMS over the training set
     rms = float(x[:, 6:9].pow(2).sum(1).mean().sqrt())
     gs = 1.0 / max(rms, 1e-30)
-    x[:, 6:9] *= gs
+    x[:, 6:12] *= gs  # the gradient and its 1-ring mean
     return x, torch.cat(ts), gs
 
 

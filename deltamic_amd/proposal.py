@@ -1,15 +1,15 @@
 """Learned per-vertex displacement proposals for the mesh fit.
 
-No counterpart in the reference.  A small MLP (12 -> 128 -> 128 -> 128 -> 3,
+No counterpart in the reference.  A small MLP (15 -> 128 -> 128 -> 128 -> 3,
 bf16 weights, FP32 accumulation; HIP kernels in csrc/mlp.hip) maps
 per-vertex features
 
     position (centred, scaled by the mesh RMS radius), unit vertex normal,
-    loss gradient (multiplied by nv * grad_scale: per-vertex gradients scale
-    like the vertex area ~ 1/nv, and grad_scale is calibrated on the training
-    trajectories, so the feature keeps the absolute gradient size the step
-    length depends on), and the normalised step m_hat / (sqrt(v_hat) + eps)
-    that Adam would take next
+    loss gradient and its 1-ring mean (both multiplied by nv * grad_scale:
+    per-vertex gradients scale like the vertex area ~ 1/nv, and grad_scale
+    is calibrated on the training trajectories, so the features keep the
+    absolute gradient size the step length depends on), and the normalised
+    step m_hat / (sqrt(v_hat) + eps) that Adam would take next
 
 to a displacement, which is smoothed over the 1-ring before use.  The
 network is trained on the method's own plain-Adam fitting trajectories: the
@@ -30,7 +30,7 @@ import torch
 from . import _lib
 from .fit import FitProblem, FitResult, adam_direction
 
-N_IN, N_HID, N_OUT = 12, 128, 3
+N_IN, N_HID, N_OUT = 15, 128, 3
 
 
 def vertex_face_table(faces: np.ndarray, nv: int):
@@ -74,7 +74,9 @@ class Features:
             g = grad / grad.pow(2).sum(1).mean().sqrt().clamp_min(1e-30)
         else:
             g = grad * (self.nv * self.grad_scale)
-        return torch.cat([P / rad, vn, g, adir], dim=1).contiguous()
+        lap = self.lap
+        gm = (g[lap.nbr] * lap.mask[..., None]).sum(1) * lap.invdeg[:, None]
+        return torch.cat([P / rad, vn, g, gm, adir], dim=1).contiguous()
 
     @torch.no_grad()
     def smooth(self, d: torch.Tensor, passes: int = 2) -> torch.Tensor:
@@ -122,6 +124,9 @@ class ProposalMLP:
                 "step": self.step_count}
 
     def load_state_dict(self, sd):
+        if sd["w"].numel() != self.nw or sd["b"].numel() != self.nb:
+            raise ValueError(f"proposal weights have {sd['w'].numel()} + {sd['b'].numel()} entries; "
+                             f"this network needs {self.nw} + {self.nb}")
         self.w = sd["w"].to(self.device).contiguous()
         self.b = sd["b"].to(self.device).contiguous()
         self.out_scale = float(sd["out_scale"])
@@ -184,7 +189,7 @@ class ProposalMLP:
 # ------------------------------------------------------------------ training data
 @dataclass
 class TrajectoryData:
-    x: torch.Tensor        # [rows, 12] features
+    x: torch.Tensor        # [rows, 15] features
     target: torch.Tensor   # [rows, 3] displacement over the horizon
 
 

@@ -226,7 +226,7 @@ int dmic_psf_aniso_grad(const float* prm, const float* grad_out, const float* ps
 /* ------------------------------------------------- proposal network (MLP)
  *
  * No counterpart in the reference: the learned per-vertex displacement
- * proposal (12 -> 128 -> 128 -> 128 -> 3, ReLU) of the fitting loop.  Weights
+ * proposal (15 -> 128 -> 128 -> 128 -> 3, ReLU) of the fitting loop.  Weights
  * are bf16 (packed row-major [in][out]: W1 12x128, W2 128x128, W3 128x128,
  * W4 128x3), biases fp32 (b1, b2, b3: 128, b4: 3); products accumulate in
  * FP32 on the matrix cores (v_mfma_f32_16x16x16_bf16).
@@ -243,7 +243,7 @@ int dmic_gemm_bf16(const void* A, int a_is_bf16, int64_t sa0, int64_t sa1, const
 int64_t dmic_mlp_weight_count(void);
 int64_t dmic_mlp_bias_count(void);
 
-/* y[rows][3] = MLP(x[rows][12]).  acts (optional, [3][rows][128]) keeps the
+/* y[rows][3] = MLP(x[rows][15]).  acts (optional, [3][rows][128]) keeps the
  * hidden activations for dmic_mlp_backward; without it scratch must hold
  * 2*rows*128 floats. */
 int dmic_mlp_forward(const uint16_t* w_bf16, const float* bias, const float* x, int64_t rows,

@@ -11,6 +11,8 @@ import torch
 
 pytestmark = pytest.mark.gpu
 
+from deltamic_amd.proposal import N_IN  # noqa: E402
+
 
 def bf16r(x):
     return x.to(torch.bfloat16).to(torch.float32)
@@ -68,7 +70,7 @@ def _torch_mlp(net, x):
     w = bf16r(net.w.cpu())
     b = net.b.cpu()
     x = x.cpu()
-    sizes = [(12, 128), (128, 128), (128, 128), (128, 3)]
+    sizes = [(N_IN, 128), (128, 128), (128, 128), (128, 3)]
     Ws, off = [], 0
     for i, o in sizes:
         Ws.append(w[off:off + i * o].view(i, o))
@@ -87,7 +89,7 @@ def test_mlp_forward_backward_vs_torch(device):
     net = ProposalMLP(device, seed=3)
     net.b.normal_(0, 0.1)
     g = torch.Generator(device="cpu").manual_seed(9)
-    x = torch.randn(1000, 12, generator=g).to(device)
+    x = torch.randn(1000, N_IN, generator=g).to(device)
     y, acts = net.forward(x, keep_acts=True)
     ref, Ws, bs = _torch_mlp(net, x)
     y = y.cpu()
@@ -113,7 +115,7 @@ def test_mlp_forward_backward_vs_torch(device):
 def test_mlp_training_reduces_loss_and_is_deterministic(device):
     from deltamic_amd.proposal import ProposalMLP
     g = torch.Generator(device="cpu").manual_seed(1)
-    x = torch.randn(4096, 12, generator=g).to(device)
+    x = torch.randn(4096, N_IN, generator=g).to(device)
     t = 0.02 * torch.tanh(x[:, :3] - 0.5 * x[:, 6:9])
     losses = []
     nets = []

@@ -201,8 +201,8 @@ def test_uniform_laplacian_adjoint_matches_dense_autograd():
 
 def test_proposal_features_on_a_sphere():
     """Per-vertex features: centred/scaled position, outward unit normal,
-    gradient scaled to unit RMS, Adam direction passed through; 1-ring
-    smoothing of a displacement field."""
+    gradient scaled to unit RMS, its 1-ring mean, Adam direction passed
+    through; 1-ring smoothing of a displacement field."""
     from deltamic_amd.fit import UniformLaplacian
     from deltamic_amd.proposal import Features
     u, f = syn.icosphere(3)
@@ -212,17 +212,18 @@ def test_proposal_features_on_a_sphere():
     g = torch.tensor(syn.rng(3).normal(size=u.shape), dtype=torch.float32)
     adir = torch.tensor(syn.rng(4).normal(size=u.shape), dtype=torch.float32)
     x = feats(V, 7.0 * g, adir)
-    assert x.shape == (len(u), 12)
-    P, n, gg, ad = x[:, 0:3], x[:, 3:6], x[:, 6:9], x[:, 9:12]
+    assert x.shape == (len(u), 15)
+    P, n, gg, gm, ad = x[:, 0:3], x[:, 3:6], x[:, 6:9], x[:, 9:12], x[:, 12:15]
     assert torch.allclose(P.pow(2).sum(1).mean(), torch.tensor(1.0), atol=1e-5)
     assert torch.allclose(n.norm(dim=1), torch.ones(len(u)), atol=1e-5)
     assert (torch.sum(n * torch.tensor(u, dtype=torch.float32), 1) > 0.99).all()  # outward
     assert torch.allclose(gg.pow(2).sum(1).mean(), torch.tensor(1.0), atol=1e-5)
     assert torch.equal(ad, adir)
-    # smoothing: one pass is the mean of a value and its 1-ring mean; constants are fixed
-    d1 = feats.smooth(g, passes=1)
     i = 17
     nb = lap.nbr[i][lap.mask[i] > 0]
+    assert torch.allclose(gm[i], gg[nb].mean(0), atol=1e-6)
+    # smoothing: one pass is the mean of a value and its 1-ring mean; constants are fixed
+    d1 = feats.smooth(g, passes=1)
     assert torch.allclose(d1[i], 0.5 * (g[i] + g[nb].mean(0)), atol=1e-6)
     c = torch.ones_like(g) * torch.tensor([0.3, -1.0, 2.0])
     assert torch.allclose(feats.smooth(c), c, atol=1e-6)
