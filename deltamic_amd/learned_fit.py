@@ -84,7 +84,9 @@ def evaluate(net, args, device, subdiv, k):
     adam = fit_adam(prob, steps=args.eval_steps, sync_every=1)
     target = adam.losses[-1]
     t_adam = adam.times[-1]
-    learned = fit_learned(prob, net, feats, steps=args.eval_steps, stop_loss=target)
+    learned = fit_learned(prob, net, feats, steps=args.eval_steps, stop_loss=target,
+                          patience=args.patience, cooldown=args.cooldown,
+                          cooldown_max=args.cooldown_max)
     reached = bool(learned.losses and learned.losses[-1] <= target)
     t_learned = learned.times[-1] if learned.times else float("nan")
     # wall time plain Adam needed to first reach the same loss is t_adam (its final loss)
@@ -121,6 +123,11 @@ def main(argv=None):
     ap.add_argument("--out", default="gpurun_out/learned")
     ap.add_argument("--load", default=None, help="skip training; load proposal.pt")
     ap.add_argument("--no-eval", action="store_true", help="train and save only")
+    ap.add_argument("--patience", type=int, default=4,
+                    help="rejections in a row before a proposal-free pause")
+    ap.add_argument("--cooldown", type=int, default=8, help="first pause length (iterations)")
+    ap.add_argument("--cooldown-max", type=int, default=256,
+                    help="pauses double up to this length")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
     device = torch.device("cuda:0")
