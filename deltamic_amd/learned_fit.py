@@ -28,7 +28,7 @@ import torch
 
 from . import synthetic as syn
 from .fit import fit_adam, make_problem
-from .proposal import Features, ProposalMLP, fit_learned, trajectory_samples
+from .proposal import Features, ProposalMLP, fit_learned, onpolicy_samples, trajectory_samples
 
 
 def log(*a):
@@ -54,6 +54,24 @@ def collect(args, device):
     gs = 1.0 / max(rms, 1e-30)
     x[:, 6:12] *= gs  # the gradient and its 1-ring mean
     return x, torch.cat(ts), gs
+
+
+def collect_onpolicy(args, net, device):
+    """On-policy rows: the learned fit with ``net`` on the training seeds,
+    plain-Adam continuations of ``--horizon`` steps as targets."""
+    xs, ts = [], []
+    for k in range(args.onpolicy_fits):
+        t0 = time.perf_counter()
+        w = syn.config4(k, subdivisions=args.subdiv, box=args.box)
+        prob = make_problem(w, device)
+        feats = Features(w.faces, len(w.extra["init_verts"]), prob.lap, device,
+                         grad_scale=net.grad_scale)
+        d = onpolicy_samples(prob, net, feats, args.train_steps, args.every, args.horizon)
+        xs.append(d.x)
+        ts.append(d.target)
+        log(f"on-policy fit {k + 1}/{args.onpolicy_fits}: {d.x.shape[0]} rows "
+            f"({time.perf_counter() - t0:.1f} s)")
+    return torch.cat(xs), torch.cat(ts)
 
 
 def train(net, x, t, epochs, batch, lr, seed):
@@ -123,6 +141,11 @@ def main(argv=None):
     ap.add_argument("--out", default="gpurun_out/learned")
     ap.add_argument("--load", default=None, help="skip training; load proposal.pt")
     ap.add_argument("--no-eval", action="store_true", help="train and save only")
+    ap.add_argument("--onpolicy-fits", type=int, default=0,
+                    help="after training (or --load), run the learned fit on this many training "
+                         "seeds, take plain-Adam continuations from its states as targets, and "
+                         "train --onpolicy-epochs more epochs on those rows")
+    ap.add_argument("--onpolicy-epochs", type=int, default=20)
     ap.add_argument("--patience", type=int, default=4,
                     help="rejections in a row before a proposal-free pause")
     ap.add_argument("--cooldown", type=int, default=8, help="first pause length (iterations)")
@@ -145,6 +168,18 @@ def main(argv=None):
         t1 = time.perf_counter()
         summary["train_loss"] = train(net, x, t, args.epochs, args.batch, args.lr, seed=1)
         summary["train_s"] = time.perf_counter() - t1
+        torch.save(net.state_dict(), os.path.join(args.out, "proposal.pt"))
+    if args.onpolicy_fits > 0:
+        t0 = time.perf_counter()
+        xo, to = collect_onpolicy(args, net, device)
+        summary["onpolicy_rows"] = int(xo.shape[0])
+        summary["onpolicy_collect_s"] = time.perf_counter() - t0
+        net.step_count = 0  # fresh Adam moments for the network's own optimiser
+        net.mw.zero_(); net.vw.zero_(); net.mb.zero_(); net.vb.zero_()
+        t1 = time.perf_counter()
+        summary["onpolicy_train_loss"] = train(net, xo, to, args.onpolicy_epochs, args.batch,
+                                               args.lr, seed=2)
+        summary["onpolicy_train_s"] = time.perf_counter() - t1
         torch.save(net.state_dict(), os.path.join(args.out, "proposal.pt"))
     with open(os.path.join(args.out, "summary.json"), "w") as f:
         json.dump(summary, f, indent=1)

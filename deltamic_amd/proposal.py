@@ -238,7 +238,8 @@ def _adam_absorb(opt: torch.optim.Adam, p: torch.Tensor, g: torch.Tensor) -> Non
 def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: int = 1000,
                 lr: float = 1e-3, stop_loss: float | None = None, scale0: float = 1.0,
                 grow: float = 1.5, shrink: float = 0.5, scale_max: float = 8.0,
-                patience: int = 4, cooldown: int = 8, cooldown_max: int = 256) -> FitResult:
+                patience: int = 4, cooldown: int = 8, cooldown_max: int = 256,
+                snapshot_every: int = 0) -> FitResult:
     """Accept/reject fit.  Each iteration the network proposes V + s*d (d
     smoothed over the 1-ring; trust scale s grows after an accepted proposal,
     shrinks after a rejection).  The candidate's loss is computed with its
@@ -251,7 +252,9 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
     proposal is made for ``cooldown`` iterations, a pause that doubles (up to
     ``cooldown_max``) each time it recurs without an acceptance in between;
     the trust scale restarts from ``scale0`` after each pause.
-    ``losses[k]`` is the loss after iteration k."""
+    ``losses[k]`` is the loss after iteration k.  With ``snapshot_every``,
+    (k, vertices, gradient, Adam direction, Adam state) is kept every that many
+    iterations (on-policy training data, onpolicy_samples)."""
     Vp = problem.init_verts.clone().requires_grad_(True)
     opt = torch.optim.Adam([Vp], lr=lr)
     losses, times = [], []
@@ -264,7 +267,13 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     L, g = problem.loss_and_grad(Vp)
+    snaps = []
     for k in range(steps):
+        if snapshot_every and k % snapshot_every == 0:
+            st = opt.state.get(Vp, {})
+            snaps.append((k, Vp.detach().clone(), g.detach().clone(),
+                          adam_direction(opt, Vp, g),
+                          {key: (v.clone() if torch.is_tensor(v) else v) for key, v in st.items()}))
         took = False
         if quiet > 0:
             quiet -= 1
@@ -305,4 +314,25 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
         if stop_loss is not None and losses[-1] <= stop_loss:
             break
     return FitResult(verts=Vp.detach(), losses=losses, times=times, accepted=accepted,
-                     proposals=proposals, accepted_at=accepted_at)
+                     proposals=proposals, accepted_at=accepted_at, snapshots=snaps)
+
+
+def onpolicy_samples(problem: FitProblem, net: ProposalMLP, feats: Features, steps: int,
+                     every: int, horizon: int, lr: float = 1e-3) -> TrajectoryData:
+    """Training pairs from the states the learned fit itself visits: the fit
+    runs with ``net``; from each snapshot (vertices and Adam state) plain Adam
+    continues for ``horizon`` steps and its displacement is the target."""
+    res = fit_learned(problem, net, feats, steps=steps, lr=lr, snapshot_every=every)
+    xs, ts = [], []
+    for k, V, g, adir, st in res.snapshots:
+        Vc = V.clone().requires_grad_(True)
+        opt = torch.optim.Adam([Vc], lr=lr)
+        if st:
+            opt.state[Vc] = {key: (v.clone() if torch.is_tensor(v) else v) for key, v in st.items()}
+        for _ in range(horizon):
+            opt.zero_grad(set_to_none=True)
+            problem.loss(Vc).backward()
+            opt.step()
+        xs.append(feats(V, g, adir))
+        ts.append(Vc.detach() - V)
+    return TrajectoryData(torch.cat(xs), torch.cat(ts))
