@@ -30,11 +30,15 @@
 //
 // sin and cos are the hardware v_sin_f32 / v_cos_f32, which take revolutions
 // (measured on MI355X: 1.2e-7 absolute, 8.7e-7 relative down to 1e-9
-// revolutions).  Their input domain is |u| <= 256 revolutions.  Faces whose
-// phase gap D exceeds kWidePhase take the "wide" variant, which reduces every
-// argument to [-1/2, 1/2] revolutions first (the median-vertex phase in
-// FP64); the others (|mid| <= D, so |x_mid| < 0.5 + D/2pi and |y/2| <= D/2
-// stay in the domain) skip the reduction.
+// revolutions).  Their input domain is |u| <= 256 revolutions.  Without a
+// reduction the FP32 arguments lose absolute precision as the phase gap D
+// grows (|mid| <= D, so |x_mid| < 1 + D/2pi and |y/4pi| <= D/4pi
+// revolutions, each rounded to half an ulp), so a wave in which any pair has
+// D > kWidePhase takes the "wide" variant for all its lanes: the
+// median-vertex phase in FP64 and every sin/cos argument reduced to
+// [-1/2, 1/2] revolutions first.  Below kWidePhase = 16 rad the unreduced
+// arguments stay under 3.6 revolutions, i.e. at most 1.2e-7 revolutions of
+// rounding, the size of the hardware sin/cos error.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -46,7 +50,7 @@ constexpr float kInv4PiF = 0.0795774715459476678844418816862571810f;
 constexpr float kInv4PiLoF = 3.2103191632826267e-09f;  // 1/(4 pi) - kInv4PiF
 constexpr float kTaylorD = 0.1f;    // largest phase gap handled by the Taylor branch
 constexpr float kPhiSeries = 1.0f;   // |y| below which Im phi'(y) uses its series
-constexpr float kWidePhase = 1000.0f;  // phase gap above which arguments are range-reduced
+constexpr float kWidePhase = 16.0f;  // phase gap above which a wave range-reduces its arguments
 // P and Q are kept at least kTinyY away from 0 (phi moves by < 7e-9 there),
 // so one reciprocal of P Q D gives 1/P, 1/Q and 1/D.
 constexpr float kTinyY = 1.3e-8f;
@@ -77,7 +81,12 @@ __device__ __forceinline__ cf expmi_rev(float u) {
   return {__builtin_amdgcn_cosf(u), -__builtin_amdgcn_sinf(u)};
 }
 // Phase of the median vertex in revolutions, reduced: FP32 sum for ordinary
-// faces, FP64 sum and exact reduction for wide ones (|mid| up to any size).
+// waves, FP64 sum and exact reduction for wide ones (|mid| up to any size).
+// Whether a wave is wide is decided for the whole wave (wave_is_wide), so
+// the two variants never run side by side under an execution mask.
+__device__ __forceinline__ bool wave_is_wide(float D) {
+  return __builtin_amdgcn_ballot_w64(D > kWidePhase) != 0;
+}
 template <bool kWide>
 __device__ __forceinline__ float mid_rev(double rev_a, float mid) {
   if (kWide) return reduce_rev(fma((double)mid, kInv2Pi, rev_a));
@@ -145,10 +154,14 @@ template <bool kWide>
 __device__ __forceinline__ void half_sincos(float y, float& s, float& c) {
   float u = y * kInv4PiF;
   if (kWide) {
-    // y/(4 pi) with the rounding error of the product and of the constant
-    // restored, then reduced: accurate to ~3e-8 revolutions for any y
-    const float lo = __builtin_fmaf(y, kInv4PiF, -u) + y * kInv4PiLoF;
-    u = (u - __builtin_rintf(u)) + lo;
+    // y/(4 pi) - r, r the nearest integer, from the exact product in one
+    // fma (|result| <= 1/2 + ulp, so one rounding of <= 3e-8), plus the
+    // error of the FP32 constant: accurate to ~6e-8 revolutions for any y.
+    // Written as explicit fmas: with the product rounded first and its error
+    // added separately, contraction of (u - r) into an fma would count that
+    // error twice.
+    const float r = __builtin_rintf(u);
+    u = __builtin_fmaf(y, kInv4PiLoF, __builtin_fmaf(y, kInv4PiF, -r));
   }
   s = __builtin_amdgcn_sinf(u);
   c = __builtin_amdgcn_cosf(u);

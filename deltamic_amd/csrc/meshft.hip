@@ -169,6 +169,70 @@ __global__ __launch_bounds__(256) void face_pack_kernel(
 
 // -------------------------------------------------------------------- forward
 
+// Squared bound on a face's phase gap per unit |xi|:
+// D = max(|xi.eb|, |xi.ec|, |xi.(ec - eb)|) <= |xi| max(|eb|, |ec|, |ec - eb|).
+__device__ __forceinline__ float face_extent2(float b0, float b1, float b2, float c0, float c1,
+                                              float c2) {
+  const float d0 = c0 - b0, d1 = c1 - b1, d2 = c2 - b2;
+  const float nb = b0 * b0 + b1 * b1 + b2 * b2;
+  const float nc = c0 * c0 + c1 * c1 + c2 * c2;
+  const float nd = d0 * d0 + d1 * d1 + d2 * d2;
+  return __builtin_fmaxf(nb, __builtin_fmaxf(nc, nd));
+}
+
+// Maximum of v over the 256 threads of the workgroup, returned to every thread.
+__device__ __forceinline__ float block_max256(float v, float* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = __builtin_fmaxf(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return __builtin_fmaxf(__builtin_fmaxf(red[0], red[1]), __builtin_fmaxf(red[2], red[3]));
+}
+
+// A pair of a tile can exceed kWidePhase only if (max |xi| of the block)^2
+// times (max face extent of the tile)^2 does.  Tiles where it cannot skip
+// the per-pair wave check (may_be_wide = false, a workgroup-uniform branch);
+// the others check every pair (wave_is_wide).
+constexpr float kWidePhase2 = kWidePhase * kWidePhase;
+
+// One LDS tile of the forward: this lane's frequency against cnt faces.
+template <bool kAF>
+__device__ __forceinline__ void fwd_tile(const FaceRec* tile, int cnt, bool may_be_wide, float x,
+                                         float y, float z, double X, double Y, double Z,
+                                         double& accr, double& acci, double& acnr, double& acni) {
+  for (int k0 = 0; k0 < cnt; k0 += kFold) {
+    const int k1 = (k0 + kFold < cnt) ? k0 + kFold : cnt;
+    float sr = 0.0f, si = 0.0f, tr = 0.0f, ti = 0.0f;
+    for (int k = k0; k < k1; ++k) {
+      const FaceRec& r = tile[k];
+      const double rev_a = fma(r.a[2], Z, fma(r.a[1], Y, r.a[0] * X));
+      const float p = __builtin_fmaf(r.eb[2], z, __builtin_fmaf(r.eb[1], y, r.eb[0] * x));
+      const float q = __builtin_fmaf(r.ec[2], z, __builtin_fmaf(r.ec[1], y, r.ec[0] * x));
+      const Sorted so = sort3(p, q);
+      cf F;   // F = e_mid N, face value = m F
+      float m;
+      if (!may_be_wide || !wave_is_wide(so.D))
+        F = cmul(expmi_rev(mid_rev<false>(rev_a, so.mid)), face_H<false>(so, m));
+      else
+        F = cmul(expmi_rev(mid_rev<true>(rev_a, so.mid)), face_H<true>(so, m));
+      const float wm = r.w * m;
+      sr = __builtin_fmaf(wm, F.re, sr);
+      si = __builtin_fmaf(wm, F.im, si);
+      if (kAF) {
+        const float tm = __builtin_fmaf(r.an[2], z, __builtin_fmaf(r.an[1], y, r.an[0] * x)) * m;
+        tr = __builtin_fmaf(tm, F.re, tr);
+        ti = __builtin_fmaf(tm, F.im, ti);
+      }
+    }
+    accr += (double)sr;
+    acci += (double)si;
+    if (kAF) {
+      acnr += (double)tr;
+      acni += (double)ti;
+    }
+  }
+}
+
 // kAF: autofluorescence variant (fourier_from_mesh_narrowband_autofluorescence,
 // render.py:228-318): each face weighs w_t + i (xi . an_t) / |xi|^2, so the
 // lane also sums (xi . an_t) F_t and adds i/|xi|^2 times it at the end; at
@@ -180,6 +244,7 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
     const float* __restrict__ xi, int64_t n, float2* __restrict__ out,
     double2* __restrict__ part, const double* __restrict__ S_in, float regint) {
   __shared__ FaceRec tile[kTile];
+  __shared__ float red[4];
   const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool active = m < n;
   float x = 0.0f, y = 0.0f, z = 0.0f;
@@ -189,45 +254,22 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
     z = xi[3 * m + 2];
   }
   const double X = x, Y = y, Z = z;
+  const float x2max = block_max256(x * x + y * y + z * z, red);
   const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
   const int64_t f1 = (f0 + faces_per_split < nf) ? f0 + faces_per_split : nf;
   double accr = 0.0, acci = 0.0, acnr = 0.0, acni = 0.0;
   for (int64_t base = f0; base < f1; base += kTile) {
     const int cnt = (int)((f1 - base < kTile) ? f1 - base : kTile);
     __syncthreads();
-    if ((int)threadIdx.x < cnt) tile[threadIdx.x] = faces[base + threadIdx.x];
-    __syncthreads();
-    for (int k0 = 0; k0 < cnt; k0 += kFold) {
-      const int k1 = (k0 + kFold < cnt) ? k0 + kFold : cnt;
-      float sr = 0.0f, si = 0.0f, tr = 0.0f, ti = 0.0f;
-      for (int k = k0; k < k1; ++k) {
-        const FaceRec& r = tile[k];
-        const double rev_a = fma(r.a[2], Z, fma(r.a[1], Y, r.a[0] * X));
-        const float p = __builtin_fmaf(r.eb[2], z, __builtin_fmaf(r.eb[1], y, r.eb[0] * x));
-        const float q = __builtin_fmaf(r.ec[2], z, __builtin_fmaf(r.ec[1], y, r.ec[0] * x));
-        const Sorted so = sort3(p, q);
-        cf F;   // F = e_mid N, face value = m F
-        float m;
-        if (so.D <= kWidePhase)
-          F = cmul(expmi_rev(mid_rev<false>(rev_a, so.mid)), face_H<false>(so, m));
-        else
-          F = cmul(expmi_rev(mid_rev<true>(rev_a, so.mid)), face_H<true>(so, m));
-        const float wm = r.w * m;
-        sr = __builtin_fmaf(wm, F.re, sr);
-        si = __builtin_fmaf(wm, F.im, si);
-        if (kAF) {
-          const float tm = __builtin_fmaf(r.an[2], z, __builtin_fmaf(r.an[1], y, r.an[0] * x)) * m;
-          tr = __builtin_fmaf(tm, F.re, tr);
-          ti = __builtin_fmaf(tm, F.im, ti);
-        }
-      }
-      accr += (double)sr;
-      acci += (double)si;
-      if (kAF) {
-        acnr += (double)tr;
-        acni += (double)ti;
-      }
+    int may_be_wide = 0;
+    if ((int)threadIdx.x < cnt) {
+      const FaceRec r = faces[base + threadIdx.x];
+      tile[threadIdx.x] = r;
+      may_be_wide = face_extent2(r.eb[0], r.eb[1], r.eb[2], r.ec[0], r.ec[1], r.ec[2]) * x2max >
+                    kWidePhase2;
     }
+    const bool tile_may_be_wide = __syncthreads_or(may_be_wide) != 0;
+    fwd_tile<kAF>(tile, cnt, tile_may_be_wide, x, y, z, X, Y, Z, accr, acci, acnr, acni);
   }
   if (kAF && active) {
     const double q = X * X + Y * Y + Z * Z;
@@ -270,6 +312,82 @@ __global__ __launch_bounds__(256) void fwd_split_reduce_kernel(const double2* __
 template <bool kAF>
 constexpr int kFaceSums = kAF ? 13 : 10;
 
+// This lane's face in the adjoint (zeros for lanes past the last face).
+struct FaceLane {
+  double A0, A1, A2;            // vertex a / (2 pi)
+  float b0, b1, b2, c0, c1, c2;  // edge vectors v_b - v_a, v_c - v_a
+  float fw;                      // face weight
+  float n0, n1, n2;              // autofluorescence normal term
+};
+
+// One LDS tile of the adjoint: this lane's face against cnt frequencies,
+// adding to the face's K sums (may_be_wide as in fwd_tile).
+template <bool kAF>
+__device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, bool may_be_wide,
+                                         const FaceLane& L, double (&acc)[kFaceSums<kAF>]) {
+  constexpr int K = kFaceSums<kAF>;
+  for (int k0 = 0; k0 < cnt; k0 += kFold) {
+    const int k1 = (k0 + kFold < cnt) ? k0 + kFold : cnt;
+    float a[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) a[i] = 0.0f;
+    for (int k = k0; k < k1; ++k) {
+      const FreqRec& fr = tile[k];
+      const double rev_a = fma(L.A2, fr.X[2], fma(L.A1, fr.X[1], L.A0 * fr.X[0]));
+      const float p =
+          __builtin_fmaf(L.b2, fr.x[2], __builtin_fmaf(L.b1, fr.x[1], L.b0 * fr.x[0]));
+      const float q =
+          __builtin_fmaf(L.c2, fr.x[2], __builtin_fmaf(L.c1, fr.x[1], L.c0 * fr.x[0]));
+      const Sorted so = sort3(p, q);
+      cf em, d, pP, pQ;
+      float rD;
+      if (!may_be_wide || !wave_is_wide(so.D)) {
+        em = expmi_rev(mid_rev<false>(rev_a, so.mid));
+        face_H_grad<false>(so, d, pP, pQ, rD);
+      } else {
+        em = expmi_rev(mid_rev<true>(rev_a, so.mid));
+        face_H_grad<true>(so, d, pP, pQ, rD);
+      }
+      // Contract with conj(g) first: z = conj(g) e_mid, then with H = d rD
+      //   rho = Re(conj g F) = Re(z H),  gamma = Re(conj g (-iF)) = Im(z H),
+      //   Re(conj g dF/dx_lo) = (Re(z pP) + rho) rD,
+      //   Re(conj g dF/dx_hi) = -(Re(z pQ) + rho) rD,
+      //   Re(conj g dF/dx_mid) = gamma - (lo) - (hi).
+      const cf z = cmul({fr.gr, -fr.gi}, em);
+      float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im) * rD;
+      cf zt = z;      // conj(g) e_mid times the face weight when it sits inside the sum
+      float rt = rh;  // Re(zt H)
+      if (kAF) {
+        // z2 = i conj(g) e_mid / |xi|^2; zt = w z + s z2 with s = xi . an
+        const cf z2 = {-z.im * fr.inv2, z.re * fr.inv2};
+        const float sn = __builtin_fmaf(L.n2, fr.x[2], __builtin_fmaf(L.n1, fr.x[1], L.n0 * fr.x[0]));
+        zt = {__builtin_fmaf(L.fw, z.re, sn * z2.re), __builtin_fmaf(L.fw, z.im, sn * z2.im)};
+        rt = __builtin_fmaf(zt.re, d.re, -zt.im * d.im) * rD;
+        const float nn = __builtin_fmaf(z2.re, d.re, -z2.im * d.im) * rD;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) a[10 + j] = __builtin_fmaf(nn, fr.x[j], a[10 + j]);
+      }
+      const float ga = __builtin_fmaf(zt.re, d.im, zt.im * d.re) * rD;
+      const float alo = (__builtin_fmaf(zt.re, pP.re, -zt.im * pP.im) + rt) * rD;
+      const float ahi = -(__builtin_fmaf(zt.re, pQ.re, -zt.im * pQ.im) + rt) * rD;
+      const float amid = ga - alo - ahi;
+      const int ra = (p < 0.0f) + (q < 0.0f);
+      const int rb = (p >= 0.0f) + (q < p);
+      const float al = (ra == 0) ? alo : ((ra == 2) ? ahi : amid);
+      const float be = (rb == 0) ? alo : ((rb == 2) ? ahi : amid);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        a[j] = __builtin_fmaf(al, fr.x[j], a[j]);
+        a[3 + j] = __builtin_fmaf(be, fr.x[j], a[3 + j]);
+        a[6 + j] = __builtin_fmaf(ga, fr.x[j], a[6 + j]);
+      }
+      a[9] += rh;
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) acc[i] += a[i];
+  }
+}
+
 template <bool kAF>
 __global__ __launch_bounds__(256) void meshft_bwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, const float* __restrict__ xi,
@@ -277,19 +395,19 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
     double* __restrict__ part /* [nchunk][kFaceSums][nf] */) {
   constexpr int K = kFaceSums<kAF>;
   __shared__ FreqRec tile[kTile];
+  __shared__ float red[4];
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool active = t < nf;
-  double A0 = 0.0, A1 = 0.0, A2 = 0.0;
-  float b0 = 0.0f, b1 = 0.0f, b2 = 0.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-  float fw = 0.0f, n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+  FaceLane L = {};
   if (active) {
     const FaceRec r = faces[t];
-    A0 = r.a[0]; A1 = r.a[1]; A2 = r.a[2];
-    b0 = r.eb[0]; b1 = r.eb[1]; b2 = r.eb[2];
-    c0 = r.ec[0]; c1 = r.ec[1]; c2 = r.ec[2];
-    fw = r.w;
-    n0 = r.an[0]; n1 = r.an[1]; n2 = r.an[2];
+    L.A0 = r.a[0]; L.A1 = r.a[1]; L.A2 = r.a[2];
+    L.b0 = r.eb[0]; L.b1 = r.eb[1]; L.b2 = r.eb[2];
+    L.c0 = r.ec[0]; L.c1 = r.ec[1]; L.c2 = r.ec[2];
+    L.fw = r.w;
+    L.n0 = r.an[0]; L.n1 = r.an[1]; L.n2 = r.an[2];
   }
+  const float e2max = block_max256(face_extent2(L.b0, L.b1, L.b2, L.c0, L.c1, L.c2), red);
   const int64_t m0 = (int64_t)blockIdx.y * freqs_per_chunk;
   const int64_t m1 = (m0 + freqs_per_chunk < n) ? m0 + freqs_per_chunk : n;
   double acc[K];
@@ -298,6 +416,7 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
   for (int64_t base = m0; base < m1; base += kTile) {
     const int cnt = (int)((m1 - base < kTile) ? m1 - base : kTile);
     __syncthreads();
+    int may_be_wide = 0;
     if ((int)threadIdx.x < cnt) {
       const int64_t m = base + threadIdx.x;
       FreqRec fr;
@@ -308,66 +427,10 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
       const double q = fr.X[0] * fr.X[0] + fr.X[1] * fr.X[1] + fr.X[2] * fr.X[2];
       fr.inv2 = (kAF && q > 0.0) ? (float)(1.0 / q) : 0.0f;
       tile[threadIdx.x] = fr;
+      may_be_wide = (float)q * e2max > kWidePhase2;
     }
-    __syncthreads();
-    for (int k0 = 0; k0 < cnt; k0 += kFold) {
-      const int k1 = (k0 + kFold < cnt) ? k0 + kFold : cnt;
-      float a[K];
-#pragma unroll
-      for (int i = 0; i < K; ++i) a[i] = 0.0f;
-      for (int k = k0; k < k1; ++k) {
-        const FreqRec& fr = tile[k];
-        const double rev_a = fma(A2, fr.X[2], fma(A1, fr.X[1], A0 * fr.X[0]));
-        const float p = __builtin_fmaf(b2, fr.x[2], __builtin_fmaf(b1, fr.x[1], b0 * fr.x[0]));
-        const float q = __builtin_fmaf(c2, fr.x[2], __builtin_fmaf(c1, fr.x[1], c0 * fr.x[0]));
-        const Sorted so = sort3(p, q);
-        cf em, d, pP, pQ;
-        float rD;
-        if (so.D <= kWidePhase) {
-          em = expmi_rev(mid_rev<false>(rev_a, so.mid));
-          face_H_grad<false>(so, d, pP, pQ, rD);
-        } else {
-          em = expmi_rev(mid_rev<true>(rev_a, so.mid));
-          face_H_grad<true>(so, d, pP, pQ, rD);
-        }
-        // Contract with conj(g) first: z = conj(g) e_mid, then with H = d rD
-        //   rho = Re(conj g F) = Re(z H),  gamma = Re(conj g (-iF)) = Im(z H),
-        //   Re(conj g dF/dx_lo) = (Re(z pP) + rho) rD,
-        //   Re(conj g dF/dx_hi) = -(Re(z pQ) + rho) rD,
-        //   Re(conj g dF/dx_mid) = gamma - (lo) - (hi).
-        const cf z = cmul({fr.gr, -fr.gi}, em);
-        float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im) * rD;
-        cf zt = z;      // conj(g) e_mid times the face weight when it sits inside the sum
-        float rt = rh;  // Re(zt H)
-        if (kAF) {
-          // z2 = i conj(g) e_mid / |xi|^2; zt = w z + s z2 with s = xi . an
-          const cf z2 = {-z.im * fr.inv2, z.re * fr.inv2};
-          const float sn = __builtin_fmaf(n2, fr.x[2], __builtin_fmaf(n1, fr.x[1], n0 * fr.x[0]));
-          zt = {__builtin_fmaf(fw, z.re, sn * z2.re), __builtin_fmaf(fw, z.im, sn * z2.im)};
-          rt = __builtin_fmaf(zt.re, d.re, -zt.im * d.im) * rD;
-          const float nn = __builtin_fmaf(z2.re, d.re, -z2.im * d.im) * rD;
-#pragma unroll
-          for (int j = 0; j < 3; ++j) a[10 + j] = __builtin_fmaf(nn, fr.x[j], a[10 + j]);
-        }
-        const float ga = __builtin_fmaf(zt.re, d.im, zt.im * d.re) * rD;
-        const float alo = (__builtin_fmaf(zt.re, pP.re, -zt.im * pP.im) + rt) * rD;
-        const float ahi = -(__builtin_fmaf(zt.re, pQ.re, -zt.im * pQ.im) + rt) * rD;
-        const float amid = ga - alo - ahi;
-        const int ra = (p < 0.0f) + (q < 0.0f);
-        const int rb = (p >= 0.0f) + (q < p);
-        const float al = (ra == 0) ? alo : ((ra == 2) ? ahi : amid);
-        const float be = (rb == 0) ? alo : ((rb == 2) ? ahi : amid);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          a[j] = __builtin_fmaf(al, fr.x[j], a[j]);
-          a[3 + j] = __builtin_fmaf(be, fr.x[j], a[3 + j]);
-          a[6 + j] = __builtin_fmaf(ga, fr.x[j], a[6 + j]);
-        }
-        a[9] += rh;
-      }
-#pragma unroll
-      for (int i = 0; i < K; ++i) acc[i] += a[i];
-    }
+    const bool tile_may_be_wide = __syncthreads_or(may_be_wide) != 0;
+    bwd_tile<kAF>(tile, cnt, tile_may_be_wide, L, acc);
   }
   if (active) {
     double* dst = part + (int64_t)blockIdx.y * K * nf + t;
