@@ -104,7 +104,7 @@ def evaluate(net, args, device, subdiv, k):
     t_adam = adam.times[-1]
     learned = fit_learned(prob, net, feats, steps=args.eval_steps, stop_loss=target,
                           patience=args.patience, cooldown=args.cooldown,
-                          cooldown_max=args.cooldown_max)
+                          cooldown_max=args.cooldown_max, accept_margin=args.accept_margin)
     reached = bool(learned.losses and learned.losses[-1] <= target)
     t_learned = learned.times[-1] if learned.times else float("nan")
     # wall time plain Adam needed to first reach the same loss is t_adam (its final loss)
@@ -149,6 +149,9 @@ def main(argv=None):
     ap.add_argument("--patience", type=int, default=4,
                     help="rejections in a row before a proposal-free pause")
     ap.add_argument("--cooldown", type=int, default=8, help="first pause length (iterations)")
+    ap.add_argument("--accept-margin", type=float, default=1.0,
+                    help="accept a proposal only if it beats this many times the first-order "
+                         "decrease of the Adam step it replaces")
     ap.add_argument("--cooldown-max", type=int, default=256,
                     help="pauses double up to this length")
     args = ap.parse_args(argv)

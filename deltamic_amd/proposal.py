@@ -239,13 +239,14 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
                 lr: float = 1e-3, stop_loss: float | None = None, scale0: float = 1.0,
                 grow: float = 1.5, shrink: float = 0.5, scale_max: float = 8.0,
                 patience: int = 4, cooldown: int = 8, cooldown_max: int = 256,
-                snapshot_every: int = 0) -> FitResult:
+                snapshot_every: int = 0, accept_margin: float = 1.0) -> FitResult:
     """Accept/reject fit.  Each iteration the network proposes V + s*d (d
     smoothed over the 1-ring; trust scale s grows after an accepted proposal,
     shrinks after a rejection).  The candidate's loss is computed with its
     autograd graph and accepted only if it is lower than the current loss by
-    more than the first-order decrease of the Adam step that would otherwise
-    be taken; its gradient then comes from that same graph (one forward + one
+    more than ``accept_margin`` times the first-order decrease of the Adam
+    step that would otherwise be taken; its gradient then comes from that
+    same graph (one forward + one
     adjoint, the cost of an Adam step) and Adam's moments absorb the gradient
     of the point left behind.  Otherwise the candidate is discarded and the
     plain Adam step is taken.  After ``patience`` rejections in a row no
@@ -283,7 +284,7 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
             proposals += 1
             cand = (Vp.detach() + s * d).requires_grad_(True)
             Lc = problem.loss(cand)
-            bar = float(L) + _adam_predicted_decrease(opt, Vp, g, adir)
+            bar = float(L) + accept_margin * _adam_predicted_decrease(opt, Vp, g, adir)
             if float(Lc.detach()) < bar:
                 (gc,) = torch.autograd.grad(Lc, cand)
                 _adam_absorb(opt, Vp, g)
