@@ -239,7 +239,8 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
                 lr: float = 1e-3, stop_loss: float | None = None, scale0: float = 1.0,
                 grow: float = 1.5, shrink: float = 0.5, scale_max: float = 8.0,
                 patience: int = 4, cooldown: int = 8, cooldown_max: int = 256,
-                snapshot_every: int = 0, accept_margin: float = 1.0) -> FitResult:
+                snapshot_every: int = 0, accept_margin: float = 8.0,
+                precond=None) -> FitResult:
     """Accept/reject fit.  Each iteration the network proposes V + s*d (d
     smoothed over the 1-ring; trust scale s grows after an accepted proposal,
     shrinks after a rejection).  The candidate's loss is computed with its
@@ -252,7 +253,10 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
     plain Adam step is taken.  After ``patience`` rejections in a row no
     proposal is made for ``cooldown`` iterations, a pause that doubles (up to
     ``cooldown_max``) each time it recurs without an acceptance in between;
-    the trust scale restarts from ``scale0`` after each pause.
+    the trust scale restarts from ``scale0`` after each pause.  With
+    ``precond`` (an optim.LargeStepsPreconditioner) the smoothed proposal is
+    further multiplied by (I + lambda L)^-2, which keeps its smooth
+    components and damps the rest.
     ``losses[k]`` is the loss after iteration k.  With ``snapshot_every``,
     (k, vertices, gradient, Adam direction, Adam state) is kept every that many
     iterations (on-policy training data, onpolicy_samples)."""
@@ -281,6 +285,8 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
         else:
             adir = adam_direction(opt, Vp, g)
             d = feats.smooth(net.propose(feats(Vp.detach(), g, adir)))
+            if precond is not None:
+                d = precond.solve(d)
             proposals += 1
             cand = (Vp.detach() + s * d).requires_grad_(True)
             Lc = problem.loss(cand)

@@ -80,7 +80,8 @@ def test_learned_fit_never_accepts_an_increase(device):
 def test_trained_proposal_gains_over_plain_adam(device):
     """The network trained on the method's own Adam trajectories (weights in
     deltamic_amd/weights/) reaches plain Adam's 200-step loss on a held-out
-    shape in fewer iterations and less wall time than plain Adam needs."""
+    shape in fewer iterations and less wall time than plain Adam needs to
+    first reach it."""
     import os
     from deltamic_amd.fit import fit_adam
     from deltamic_amd.proposal import Features, ProposalMLP, fit_learned
@@ -98,13 +99,15 @@ def test_trained_proposal_gains_over_plain_adam(device):
     prob.loss_and_grad(prob.init_verts)
     adam = fit_adam(prob, steps=200)
     target = adam.losses[-1]
+    # Adam's time to the target is when it first reached it (its loss need not be monotone)
+    first = next(i for i, v in enumerate(adam.losses) if v <= target)
     learned = fit_learned(prob, net, feats, steps=200, stop_loss=target)
-    print(f"held-out shape: Adam 200 steps {adam.times[-1]:.2f} s -> {target:.6f}; learned "
+    print(f"held-out shape: Adam {first} steps {adam.times[first]:.2f} s -> {target:.6f}; learned "
           f"{len(learned.losses)} it, {learned.times[-1]:.2f} s -> {learned.losses[-1]:.6f} "
           f"({learned.accepted} accepted)")
     assert learned.losses[-1] <= target
-    assert len(learned.losses) < 200
-    assert learned.times[-1] < adam.times[-1]
+    assert len(learned.losses) < first
+    assert learned.times[-1] < adam.times[first]
 
 
 def test_large_steps_fit_decreases_loss(device):
