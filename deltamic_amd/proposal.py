@@ -15,8 +15,11 @@ to a displacement, which is smoothed over the 1-ring before use.  The
 network is trained on the method's own plain-Adam fitting trajectories: the
 target for a snapshot (V_k, grad_k, Adam state_k) is the displacement Adam
 makes over the next ``horizon`` steps.  In the fit, a proposal V + d is
-accepted only if the meshFT + OTF loss decreases by more than the Adam step
-would decrease it; otherwise a plain Adam step is taken (fit_learned).
+accepted only if the meshFT + OTF loss decreases by more than 8 times the
+first-order decrease of the Adam step it replaces; otherwise a plain Adam
+step is taken and the proposal's trust scale halves (fit_learned).  A
+margin of 1 accepts proposals that barely beat an Adam step and whose
+jumps the following Adam steps partly undo; 8 keeps only large gains.
 """
 from __future__ import annotations
 
