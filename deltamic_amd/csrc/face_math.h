@@ -54,6 +54,16 @@ constexpr float kWidePhase = 16.0f;  // phase gap above which a wave range-reduc
 // P and Q are kept at least kTinyY away from 0 (phi moves by < 7e-9 there),
 // so one reciprocal of P Q D gives 1/P, 1/Q and 1/D.
 constexpr float kTinyY = 1.3e-8f;
+constexpr float kFltMax = 3.402823466e38f;
+
+// P <= -kTinyY and Q >= kTinyY, each one v_med3_f32 (fminf/fmaxf would add a
+// NaN-quieting canonicalize per operand).
+__device__ __forceinline__ float clamp_P(float P) {
+  return __builtin_amdgcn_fmed3f(P, -kFltMax, -kTinyY);
+}
+__device__ __forceinline__ float clamp_Q(float Q) {
+  return __builtin_amdgcn_fmed3f(Q, kTinyY, kFltMax);
+}
 
 struct cf {
   float re, im;
@@ -179,7 +189,7 @@ __device__ __forceinline__ cf face_H(const Sorted& so, float& m) {
     N = taylor_H(so.P, so.Q);
     m = 1.0f;
   } else {
-    const float P = __builtin_fminf(so.P, -kTinyY), Q = __builtin_fmaxf(so.Q, kTinyY);
+    const float P = clamp_P(so.P), Q = clamp_Q(so.Q);
     float sP, cP, sQ, cQ;
     half_sincos<kWide>(P, sP, cP);
     half_sincos<kWide>(Q, sQ, cQ);
@@ -205,7 +215,7 @@ __device__ __forceinline__ void face_H_grad(const Sorted& so, cf& d, cf& pP, cf&
     pQ = {-tQ.re - d.re, -tQ.im - d.im};
     rD = 1.0f;
   } else {
-    const float P = __builtin_fminf(so.P, -kTinyY), Q = __builtin_fmaxf(so.Q, kTinyY);
+    const float P = clamp_P(so.P), Q = clamp_Q(so.Q);
     float sP, cP, sQ, cQ;
     half_sincos<kWide>(P, sP, cP);
     half_sincos<kWide>(Q, sQ, cQ);
