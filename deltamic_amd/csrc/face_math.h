@@ -176,18 +176,18 @@ __device__ __forceinline__ void half_sincos(float y, float& s, float& c) {
   s = __builtin_amdgcn_sinf(u);
   c = __builtin_amdgcn_cosf(u);
 }
-// H(P, Q) for the forward, returned as H = m * N (the kernel folds m into
-// the face weight):
+// H(P, Q) for the forward, returned as H = -2 m N (the kernel folds m into
+// the face weight, which carries the factor -2):
 //     H = (phi(P) - phi(Q))/D = (Q A_P - P A_Q) / (P Q D),
 //     A_y = exp(-iy) - 1 = -2 s (s + i c),  s = sin(y/2), c = cos(y/2),
-// so N = sP Q (sP + i cP) - sQ P (sQ + i cQ) and m = -2/(P Q D): one
-// reciprocal per pair.
+// so N = sP Q (sP + i cP) - sQ P (sQ + i cQ) and m = 1/(P Q D): one
+// reciprocal per pair.  The Taylor branch returns N = H and m = -1/2.
 template <bool kWide>
 __device__ __forceinline__ cf face_H(const Sorted& so, float& m) {
   cf N;
   if (so.D < kTaylorD) {
     N = taylor_H(so.P, so.Q);
-    m = 1.0f;
+    m = -0.5f;
   } else {
     const float P = clamp_P(so.P), Q = clamp_Q(so.Q);
     float sP, cP, sQ, cQ;
@@ -195,7 +195,7 @@ __device__ __forceinline__ cf face_H(const Sorted& so, float& m) {
     half_sincos<kWide>(Q, sQ, cQ);
     const float u = sP * Q, v = sQ * P;
     N = {__builtin_fmaf(u, sP, -v * sQ), __builtin_fmaf(u, cP, -v * cQ)};
-    m = -2.0f * rcp(P * Q * so.D);
+    m = rcp(P * Q * so.D);
   }
   return N;
 }

@@ -41,8 +41,8 @@ struct __align__(16) FaceRec {
   double a[3];   // vertex a / (2 pi) in FP64: xi . a[] is the phase of a in revolutions
   float eb[3];   // v_b - v_a (FP32, correctly rounded)
   float ec[3];   // v_c - v_a
-  float w;       // 2 c_t A_t / sum c A
-  float an[3];   // autofluorescence variant: (regint / regvol) (v_b - v_a) x (v_c - v_a); else 0
+  float w;       // -2 x the face weight 2 c_t A_t / sum c A (the forward's -2 folded in)
+  float an[3];   // autofluorescence variant: -2 (regint / regvol) (v_b - v_a) x (v_c - v_a); else 0
 };
 static_assert(sizeof(FaceRec) == 64, "FaceRec layout");
 
@@ -151,10 +151,11 @@ __global__ __launch_bounds__(256) void face_pack_kernel(
     r.eb[d] = P[1][d] - P[0][d];
     r.ec[d] = P[2][d] - P[0][d];
   }
-  r.w = (S > 0.0) ? (float)(2.0 * cA[t] / S) : 0.0f;
+  r.w = (S > 0.0) ? (float)(-4.0 * cA[t] / S) : 0.0f;
   // autofluorescence: nor2_t = nor_t regint / regvol, nor_t = a x b + b x c + c x a
   // = (b - a) x (c - a) (render.py:208,232)
-  const double sc = (regint != 0.0f && regvol != 0.0) ? (double)regint / regvol : 0.0;
+  // the forward's factor -2 is folded in here too, as in r.w
+  const double sc = (regint != 0.0f && regvol != 0.0) ? -2.0 * (double)regint / regvol : 0.0;
   double u[3], q[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -203,13 +204,12 @@ __device__ __forceinline__ void fwd_tile(const FaceRec* tile, int cnt, bool may_
   for (int k0 = 0; k0 < cnt; k0 += kFold) {
     const int k1 = (k0 + kFold < cnt) ? k0 + kFold : cnt;
     float sr = 0.0f, si = 0.0f, tr = 0.0f, ti = 0.0f;
-    for (int k = k0; k < k1; ++k) {
-      const FaceRec& r = tile[k];
+    auto pair = [&](const FaceRec& r) {
       const double rev_a = fma(r.a[2], Z, fma(r.a[1], Y, r.a[0] * X));
       const float p = __builtin_fmaf(r.eb[2], z, __builtin_fmaf(r.eb[1], y, r.eb[0] * x));
       const float q = __builtin_fmaf(r.ec[2], z, __builtin_fmaf(r.ec[1], y, r.ec[0] * x));
       const Sorted so = sort3(p, q);
-      cf F;   // F = e_mid N, face value = m F
+      cf F;   // F = e_mid N, face value = -2 m F (r.w carries the -2)
       float m;
       if (!may_be_wide || !wave_is_wide(so.D))
         F = cmul(expmi_rev(mid_rev<false>(rev_a, so.mid)), face_H<false>(so, m));
@@ -223,7 +223,15 @@ __device__ __forceinline__ void fwd_tile(const FaceRec* tile, int cnt, bool may_
         tr = __builtin_fmaf(tm, F.re, tr);
         ti = __builtin_fmaf(tm, F.im, ti);
       }
+    };
+    // two faces per iteration: one LDS base address serves both records
+    // (the loop holds a wave ballot, so the compiler will not unroll it)
+    int k = k0;
+    for (; k + 2 <= k1; k += 2) {
+      pair(tile[k]);
+      pair(tile[k + 1]);
     }
+    if (k < k1) pair(tile[k]);
     accr += (double)sr;
     acci += (double)si;
     if (kAF) {
@@ -404,8 +412,8 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
     L.A0 = r.a[0]; L.A1 = r.a[1]; L.A2 = r.a[2];
     L.b0 = r.eb[0]; L.b1 = r.eb[1]; L.b2 = r.eb[2];
     L.c0 = r.ec[0]; L.c1 = r.ec[1]; L.c2 = r.ec[2];
-    L.fw = r.w;
-    L.n0 = r.an[0]; L.n1 = r.an[1]; L.n2 = r.an[2];
+    L.fw = -0.5f * r.w;  // the face weight 2 c_t A_t / sum c A (exact)
+    L.n0 = -0.5f * r.an[0]; L.n1 = -0.5f * r.an[1]; L.n2 = -0.5f * r.an[2];  // exact
   }
   const float e2max = block_max256(face_extent2(L.b0, L.b1, L.b2, L.c0, L.c1, L.c2), red);
   const int64_t m0 = (int64_t)blockIdx.y * freqs_per_chunk;
