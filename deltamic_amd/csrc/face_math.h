@@ -21,7 +21,7 @@
 //     dF/dx_lo = e_m dH/dP,  dF/dx_hi = e_m dH/dQ,
 //     dF/dx_mid = -iF - dF/dx_lo - dF/dx_hi,
 //     dH/dP = (phi'(P) + H)/D,  dH/dQ = -(phi'(Q) + H)/D,
-// with Im phi'(y) from its series for |y| < 1 and in closed form otherwise.
+// with Im phi'(y) from its series for |y| < 0.85 and in closed form otherwise.
 //
 // The phase of vertex a is computed in revolutions in FP64 from xi (FP32,
 // exact in FP64) and a / (2 pi) (FP64, relative error 1e-16), and reduced
@@ -49,7 +49,7 @@ constexpr double kInv2Pi = 0.159154943091895335768883763372514362;
 constexpr float kInv4PiF = 0.0795774715459476678844418816862571810f;
 constexpr float kInv4PiLoF = 3.2103191632826267e-09f;  // 1/(4 pi) - kInv4PiF
 constexpr float kTaylorD = 0.1f;    // largest phase gap handled by the Taylor branch
-constexpr float kPhiSeries = 1.0f;   // |y| below which Im phi'(y) uses its series
+constexpr float kPhiSeries = 0.85f;  // |y| below which Im phi'(y) uses its series
 constexpr float kWidePhase = 16.0f;  // phase gap above which a wave range-reduces its arguments
 // P and Q are kept at least kTinyY away from 0 (phi moves by < 7e-9 there),
 // so one reciprocal of P Q D gives 1/P, 1/Q and 1/D.
@@ -103,15 +103,15 @@ __device__ __forceinline__ float mid_rev(double rev_a, float mid) {
   return __builtin_fmaf(mid, kInv2PiF, reduce_rev(rev_a));
 }
 // phi'(y) = [(-sin y + k s) + i(-cos y + k c)] / y.
-// Real part = (k/2)(k - 2c): no cancellation for any y (k, c -> 1 as y -> 0).
-// Imaginary part = (k c + 2 s^2 - 1)/y cancels for small y; below kPhiSeries
-// it comes from its series y (1/3 - y^2/30 + y^4/840 - y^6/45360 +
-// y^8/3991680) (truncation < 2e-9 at |y| = 1).
+// Real part = k (k/2 - c): no cancellation for any y (k, c -> 1 as y -> 0).
+// Imaginary part = (k c + 2 s^2 - 1)/y (the spherical Bessel j1(y)) cancels
+// for small y (relative error ~5e-7/y^2 from the sin/cos); below kPhiSeries
+// it comes from its series y (1/3 - y^2/30 + y^4/840 - y^6/45360)
+// (truncation < 6e-8 at |y| = 0.85, where the closed form is within 1.5e-6).
 __device__ __forceinline__ cf dphi_of(float y, float s, float c, float k, float ry) {
-  const float re = 0.5f * k * __builtin_fmaf(-2.0f, c, k);
+  const float re = k * __builtin_fmaf(0.5f, k, -c);
   const float y2 = y * y;
-  float si = __builtin_fmaf(y2, 2.5052108385e-7f, -2.2045855379e-5f);
-  si = __builtin_fmaf(y2, si, 1.1904761905e-3f);
+  float si = __builtin_fmaf(y2, -2.2045855379e-5f, 1.1904761905e-3f);
   si = __builtin_fmaf(y2, si, -3.3333333333e-2f);
   si = __builtin_fmaf(y2, si, 3.3333333333e-1f) * y;
   const float di = __builtin_fmaf(k, c, __builtin_fmaf(2.0f * s, s, -1.0f)) * ry;
