@@ -356,28 +356,31 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, bool may_
         em = expmi_rev(mid_rev<true>(rev_a, so.mid));
         face_H_grad<true>(so, d, pP, pQ, rD);
       }
-      // Contract with conj(g) first: z = conj(g) e_mid, then with H = d rD
-      //   rho = Re(conj g F) = Re(z H),  gamma = Re(conj g (-iF)) = Im(z H),
-      //   Re(conj g dF/dx_lo) = (Re(z pP) + rho) rD,
-      //   Re(conj g dF/dx_hi) = -(Re(z pQ) + rho) rD,
+      // Contract with conj(g) first, scaled by rD: z = conj(g) e_mid rD, then
+      // with H = d rD
+      //   rho = Re(conj g F) = Re(z d),  gamma = Re(conj g (-iF)) = Im(z d),
+      //   Re(conj g dF/dx_lo) = Re(z pP) + rho rD,
+      //   Re(conj g dF/dx_hi) = -(Re(z pQ) + rho rD),
       //   Re(conj g dF/dx_mid) = gamma - (lo) - (hi).
-      const cf z = cmul({fr.gr, -fr.gi}, em);
-      float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im) * rD;
-      cf zt = z;      // conj(g) e_mid times the face weight when it sits inside the sum
-      float rt = rh;  // Re(zt H)
+      const cf e = cmul({fr.gr, -fr.gi}, em);
+      const cf z = {e.re * rD, e.im * rD};
+      const float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im);
+      cf zt = z;      // conj(g) e_mid rD times the face weight when it sits inside the sum
+      float rt = rh;  // Re(zt d)
       if (kAF) {
-        // z2 = i conj(g) e_mid / |xi|^2; zt = w z + s z2 with s = xi . an
+        // z2 = i conj(g) e_mid rD / |xi|^2; zt = w z + s z2 with s = xi . an
         const cf z2 = {-z.im * fr.inv2, z.re * fr.inv2};
         const float sn = __builtin_fmaf(L.n2, fr.x[2], __builtin_fmaf(L.n1, fr.x[1], L.n0 * fr.x[0]));
         zt = {__builtin_fmaf(L.fw, z.re, sn * z2.re), __builtin_fmaf(L.fw, z.im, sn * z2.im)};
-        rt = __builtin_fmaf(zt.re, d.re, -zt.im * d.im) * rD;
-        const float nn = __builtin_fmaf(z2.re, d.re, -z2.im * d.im) * rD;
+        rt = __builtin_fmaf(zt.re, d.re, -zt.im * d.im);
+        const float nn = __builtin_fmaf(z2.re, d.re, -z2.im * d.im);
 #pragma unroll
         for (int j = 0; j < 3; ++j) a[10 + j] = __builtin_fmaf(nn, fr.x[j], a[10 + j]);
       }
-      const float ga = __builtin_fmaf(zt.re, d.im, zt.im * d.re) * rD;
-      const float alo = (__builtin_fmaf(zt.re, pP.re, -zt.im * pP.im) + rt) * rD;
-      const float ahi = -(__builtin_fmaf(zt.re, pQ.re, -zt.im * pQ.im) + rt) * rD;
+      const float rtD = rt * rD;
+      const float ga = __builtin_fmaf(zt.re, d.im, zt.im * d.re);
+      const float alo = __builtin_fmaf(zt.re, pP.re, __builtin_fmaf(-zt.im, pP.im, rtD));
+      const float ahi = -__builtin_fmaf(zt.re, pQ.re, __builtin_fmaf(-zt.im, pQ.im, rtD));
       const float amid = ga - alo - ahi;
       const int ra = (p < 0.0f) + (q < 0.0f);
       const int rb = (p >= 0.0f) + (q < p);
