@@ -152,12 +152,14 @@ def test_backward_cfg2_subset(device):
 
 
 def test_wide_phase_faces(device):
-    """Faces whose phase gap exceeds 1000 rad take the range-reduced variant
-    (their sin/cos arguments leave the hardware domain of 256 revolutions).
+    """Faces whose phase gap exceeds 16 rad take the range-reduced variant;
+    here gaps reach thousands of rad, beyond the hardware sin/cos domain of
+    256 revolutions.
     Vertices are multiples of 1/8 and frequencies multiples of 1/4, so every
     FP32 edge vector and phase difference is exact and the usual tolerances
     apply.  Large random triangles (phase gaps of thousands of rad) are mixed
-    with small ones (tens of rad) so both variants run in the same waves."""
+    with small ones (tens of rad) in the same waves, which then take the
+    reduced arguments for all their lanes."""
     from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
     g = syn.rng(77)
     big = (g.integers(-48, 49, size=(120, 3)) / 8.0).astype(np.float32)
@@ -172,7 +174,7 @@ def test_wide_phase_faces(device):
     faces, e1, e2, nbig = faces[keep], e1[keep], e2[keep], int(keep[:96].sum())
     coeff = g.uniform(0.5, 1.5, len(faces)).astype(np.float32)
     xi = (g.integers(-1200, 1201, size=(768, 3)) / 4.0).astype(np.float32)
-    # phase gaps per pair: most large faces exceed the 1000 rad threshold
+    # phase gaps per pair: most large faces exceed 1000 rad, the small ones stay below
     p, q = xi.astype(np.float64) @ e1.T, xi.astype(np.float64) @ e2.T
     gap = np.maximum.reduce([np.abs(p), np.abs(q), np.abs(p - q)])
     assert (gap[:, :nbig] > 1000).mean() > 0.5 and (gap[:, nbig:] < 1000).all()
