@@ -61,3 +61,20 @@ def test_oracle_cube_vs_exact_vectors():
     want = surf - 0.7 * solid
     want[np.all(xi == 0, axis=1)] = 1.0 - 0.7
     assert np.max(np.abs(af - want)) <= 1e-9
+
+
+def test_cube_fixture_matches_numpy_closed_form():
+    """The committed cube vectors against the same closed form in numpy
+    (double precision, the 1-D integral written as exp(-ik/2) sin(k/2)/(k/2)
+    to avoid cancellation)."""
+    c = load("cube_surface.json")
+    xi = np.array(c["xi"])
+
+    def I(k):
+        return 1.0 if k == 0 else np.exp(-0.5j * k) * np.sin(0.5 * k) / (0.5 * k)
+
+    surf = np.array([(I(a) * I(b) * (1 + np.exp(-1j * z)) + I(b) * I(z) * (1 + np.exp(-1j * a))
+                      + I(z) * I(a) * (1 + np.exp(-1j * b))) / 6.0 for a, b, z in xi])
+    solid = np.array([I(a) * I(b) * I(z) for a, b, z in xi])
+    assert np.max(np.abs(surf - cplx(c["surface"]))) <= 1e-12
+    assert np.max(np.abs(solid - cplx(c["solid"]))) <= 1e-12
