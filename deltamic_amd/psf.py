@@ -31,6 +31,8 @@ class _GaussianPSF(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, S, x0, x1, x2):
+        if S.numel() != 9:  # the kernel reads a 3 x 3 matrix
+            raise ValueError(f"sigma must be 3 x 3, got {tuple(S.shape)}")
         axes = _gpu_axes(x0, x1, x2)
         Sd = S.detach().to(device=axes[0].device, dtype=torch.float32).contiguous()
         n0, n1, n2 = (a.numel() for a in axes)
@@ -64,6 +66,8 @@ class _AnisotropicPSF(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, prm, x0, x1, x2):
+        if prm.numel() != 6:  # the kernel reads six parameters
+            raise ValueError(f"anisotropic PSF takes 6 parameters, got {prm.numel()}")
         axes = _gpu_axes(x0, x1, x2)
         p = prm.detach().to(device=axes[0].device, dtype=torch.float32).contiguous()
         n0, n1, n2 = (a.numel() for a in axes)
@@ -102,6 +106,8 @@ def generate_gaussian_psf(sigma, xi0, xi1, xi2):
     Function whose backward reduces dL/dSigma on the GPU (the learnable-PSF
     case of the reference's fit loop)."""
     sig_t = sigma if isinstance(sigma, torch.Tensor) else torch.as_tensor(np.asarray(sigma))
+    if sig_t.numel() != 9:
+        raise ValueError(f"sigma must be 3 x 3, got {tuple(sig_t.shape)}")
     axes = _gpu_axes(xi0, xi1, xi2)
     if sig_t.requires_grad and torch.is_grad_enabled():
         return _GaussianPSF.apply(sig_t, *axes)

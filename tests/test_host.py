@@ -168,6 +168,21 @@ def test_psf_on_cpu_raises():
     assert m.x2[0] == -1.5 and m.x2[-1] == 1.5 and m.Gridxi1.shape == (4, 5, 6)
 
 
+def test_psf_rejects_wrong_sized_sigma_before_launch():
+    """The kernels read a 3 x 3 sigma (6 parameters for the anisotropic
+    PSF); other sizes are refused on the host, with or without gradients."""
+    from deltamic_amd import generate_gaussian_psf
+    from deltamic_amd.psf import _AnisotropicPSF, _GaussianPSF
+    ax = [torch.linspace(-30, 30, n) for n in (5, 6, 7)]
+    for sig in (torch.ones(4), torch.ones(2, 2, requires_grad=True), torch.ones(10)):
+        with pytest.raises(ValueError):
+            generate_gaussian_psf(sig, *ax)
+    with pytest.raises(ValueError):
+        _GaussianPSF.apply(torch.ones(4, requires_grad=True), *ax)
+    with pytest.raises(ValueError):
+        _AnisotropicPSF.apply(torch.ones(5, requires_grad=True), *ax)
+
+
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
