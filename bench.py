@@ -36,11 +36,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """(threads to use, description of the host): the threads are this GPU's
+    share of the host (OMP_NUM_THREADS, else the CPUs this process may run on)."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = int(omp) if omp and omp.isdigit() and int(omp) > 0 else affinity
+    threads = max(1, min(threads, affinity))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return threads, {"nproc": os.cpu_count(), "affinity": affinity, "omp_num_threads": omp,
+                     "model": model}
+
+
 def cpu_baseline(w, xi_np, Vs, n_sample: int):
-    """Reference-semantics FP32 restatement (oracle), fwd + adjoint, all host cores."""
+    """Reference-semantics FP32 restatement (oracle), fwd + adjoint, on this
+    GPU's share of the host cores."""
     from oracle import oracle as O
     from deltamic_amd import synthetic as syn
-    cores = min(16, os.cpu_count() or 1)
+    cores, host = host_cpu()
     sel = np.sort(syn.rng(w.seed + 1).choice(len(xi_np), n_sample, replace=False))
     xi = xi_np[sel]
     g = syn.rng(w.seed + 4)
@@ -55,6 +79,7 @@ def cpu_baseline(w, xi_np, Vs, n_sample: int):
         "value": pairs / (t2 - t0) / 1e9,
         "unit": UNIT,
         "cores": cores,
+        "host": host,
         "kind": "port",
         "sample": (f"same mesh, {n_sample} of the band frequencies (seed {w.seed + 1}); "
                    f"reference-semantics FP32 forward {t1 - t0:.2f} s + adjoint {t2 - t1:.2f} s "
@@ -108,6 +133,69 @@ def psf_leg(L, dev, w, steps):
     return res
 
 
+def render_leg(dev, w, band, otf, vals, reps):
+    """The render chain around meshFT on the bench box, op by op (render.py:5-6,
+    177-180; cuda_class.py:27-28,36): each op's average time from events on
+    the stream it runs on, its algorithmic HBM bytes (each operand read once,
+    each result written once) and the rate against 8 TB/s."""
+    import torch
+    from deltamic_amd import normalize_tensor
+    from deltamic_amd.hip_class import gather_band, scatter_band
+    nbox = int(np.prod(w.box_shape))
+    n = band.idx.numel()
+    shape = list(w.box_shape)
+    box = scatter_band(vals, band)
+    prod = box * otf
+    spec = torch.fft.ifftn(prod, shape)
+    img = torch.abs(spec)
+    up = torch.randn(nbox, device=dev).reshape(shape)
+    ops = [
+        # name, fn, algorithmic bytes
+        ("scatter_band", lambda: scatter_band(vals, band), 8 * nbox + 12 * n),
+        ("otf_multiply", lambda: box * otf, 20 * nbox),
+        ("ifftn", lambda: torch.fft.ifftn(prod, shape), 16 * nbox),
+        ("abs", lambda: torch.abs(spec), 12 * nbox),
+        ("normalize", lambda: normalize_tensor(img), 16 * nbox),
+        ("gather_band", lambda: gather_band(prod, band), 12 * n),
+    ]
+    res = {}
+    for name, fn, nbytes in ops:
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        res[name] = {"ms": round(ms, 4), "bytes": nbytes,
+                     "GBps": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    # the whole chain's backward (normalize, abs, ifftn, multiply) to the ftmesh box
+    boxg = box.clone().requires_grad_(True)
+
+    def chain_bwd():
+        out = normalize_tensor(torch.abs(torch.fft.ifftn(boxg * otf, shape)))
+        (gb,) = torch.autograd.grad(out, boxg, up)
+        return gb
+
+    chain_bwd()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        chain_bwd()
+    b.record()
+    torch.cuda.synchronize()
+    res["chain_forward_and_backward_ms"] = round(a.elapsed_time(b) / reps, 4)
+    res["box"] = list(w.box_shape)
+    res["band"] = n
+    res["hbm_peak_GBps"] = 8000.0
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +215,9 @@ def main() -> int:
     ap.add_argument("--af-steps", type=int, default=3,
                     help="also time the autofluorescence variant (forward + adjoint) on the "
                          "same mesh and band (reported under 'autofluorescence')")
+    ap.add_argument("--render-reps", type=int, default=10,
+                    help="time each op of the render chain this many times (0 = skip; "
+                         "reported under 'render')")
     ap.add_argument("--psf-steps", type=int, default=5,
                     help="also time the learnable-PSF kernels (forward and sigma gradient) on "
                          "the box (HBM-bound; reported separately under 'psf')")
@@ -317,6 +408,13 @@ def main() -> int:
               "steps": args.af_steps, "regint": 0.5}
         log(f"autofluorescence variant: {ams:.3f} ms/step, {af['value']} {UNIT}")
 
+    render = None
+    if args.render_reps > 0:
+        render = render_leg(dev, w, band, otf, out, args.render_reps)
+        log("render chain on " + str(w.box_shape) + ": " + ", ".join(
+            f"{k} {v['ms']:.4f} ms ({v['GBps']:.0f} GB/s)" for k, v in render.items()
+            if isinstance(v, dict)) + f"; chain fwd+bwd {render['chain_forward_and_backward_ms']} ms")
+
     psf = None
     if args.psf_steps > 0:
         psf = psf_leg(L, dev, w, args.psf_steps)
@@ -383,6 +481,7 @@ def main() -> int:
         },
         "full_band_evaluation": full_band,
         "autofluorescence": af,
+        "render": render,
         "psf": psf,
         "cpu_baseline": cpu,
     }

@@ -47,7 +47,8 @@ __global__ __launch_bounds__(64) void gemm_bf16_kernel(const void* __restrict__ 
                                                        const float* __restrict__ mask,
                                                        int64_t ldm, int epi) {
   const int l = threadIdx.x;
-  const int64_t m0 = (int64_t)blockIdx.y * 16, n0 = (int64_t)blockIdx.x * 16;
+  // rows (one per vertex: the large dimension) on grid.x, columns on grid.y
+  const int64_t m0 = (int64_t)blockIdx.x * 16, n0 = (int64_t)blockIdx.y * 16;
   const int64_t ar = m0 + (l & 15);
   const int64_t bc = n0 + (l & 15);
   const int kq = 4 * (l >> 4);
@@ -124,8 +125,10 @@ static int gemm(const void* A, int64_t sa0, int64_t sa1, const void* B, int64_t 
                 float* C, int64_t ldc, int64_t M, int64_t N, int64_t K, const float* bias,
                 const float* mask, int64_t ldm, int epi, hipStream_t st) {
   if (M == 0 || N == 0) return DMIC_OK;
-  DMIC_REQUIRE((M + 15) / 16 < 2147483647 && (N + 15) / 16 < 65536, DMIC_ERR_ARG, "gemm too large");
-  dim3 grid((unsigned)((N + 15) / 16), (unsigned)((M + 15) / 16));
+  DMIC_REQUIRE((M + 15) / 16 <= 2147483647 && (N + 15) / 16 <= 65535, DMIC_ERR_ARG,
+               "gemm too large: %lld x %lld output tiles (at most 2^31-1 row tiles and 65535 "
+               "column tiles)", (long long)((M + 15) / 16), (long long)((N + 15) / 16));
+  dim3 grid((unsigned)((M + 15) / 16), (unsigned)((N + 15) / 16));
   hipLaunchKernelGGL((gemm_bf16_kernel<ABF, BBF>), grid, dim3(64), 0, st, A, sa0, sa1, B, sb0, sb1,
                      C, ldc, M, N, K, bias, mask, ldm, epi);
   DMIC_LAUNCH_CHECK();

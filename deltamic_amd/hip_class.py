@@ -7,8 +7,12 @@ work done by HIP kernels:
 * band compaction ``Filter > thresh`` (no abs, cuda_class.py:21) in
   row-major voxel order without materialising a meshgrid (cuda_class.py:20-23);
 * the band is cached on the ``Filter`` tensor and rebuilt only when that
-  tensor, its version counter, the threshold or the frequency axes change
-  (the reference rebuilds it on every forward);
+  tensor object, its storage address, its version counter, the threshold or
+  the frequency axes change (the reference rebuilds it on every forward).
+  Writes through ``Filter.data`` or a raw pointer bypass the version counter:
+  after such an edit call ``clear_band_cache(Filter)`` (or
+  ``Fourier3dMesh.clear_band_cache()``), or build the module with
+  ``cache_band=False`` to recompact on every forward as the reference does;
 * scatter into the full complex box (cuda_class.py:27-28) and the gather of
   ``grad_output`` (cuda_class.py:36);
 * Hermitian halving: when the band is exactly centrosymmetric (always the
@@ -98,11 +102,14 @@ def compute_band(Filter: torch.Tensor, thresh: float, xi0: torch.Tensor, xi1: to
     return Band(idx=idx, xi_list=xi_list, shape=(n0, n1, n2), key=(), symmetric=symmetric)
 
 
-def band_for(Filter: torch.Tensor, narrowband_thresh, xi0, xi1, xi2) -> Band:
-    """Band of ``Filter``, cached on the tensor object itself."""
+def band_for(Filter: torch.Tensor, narrowband_thresh, xi0, xi1, xi2, cache: bool = True) -> Band:
+    """Band of ``Filter``, cached on the tensor object itself (``cache=False``:
+    recompacted on every call, the reference's behaviour)."""
     thresh = _thresh_value(narrowband_thresh)
-    key = (Filter._version, thresh, id(xi0), xi0._version, id(xi1), xi1._version, id(xi2),
-           xi2._version)
+    if not cache:
+        return compute_band(Filter, thresh, xi0, xi1, xi2)
+    key = (Filter.data_ptr(), Filter._version, thresh, id(xi0), xi0._version, id(xi1),
+           xi1._version, id(xi2), xi2._version)
     cached = getattr(Filter, "_dmic_band", None)
     if cached is not None and cached[0] == key and cached[2] is xi0 and cached[3] is xi1 \
             and cached[4] is xi2:
@@ -111,6 +118,12 @@ def band_for(Filter: torch.Tensor, narrowband_thresh, xi0, xi1, xi2) -> Band:
     band.key = key
     Filter._dmic_band = (key, band, xi0, xi1, xi2)
     return band
+
+
+def clear_band_cache(Filter: torch.Tensor) -> None:
+    """Drops the band cached on ``Filter``; the next forward recompacts it."""
+    if hasattr(Filter, "_dmic_band"):
+        del Filter._dmic_band
 
 
 def scatter_band(vals: torch.Tensor, band: Band) -> torch.Tensor:
@@ -163,10 +176,11 @@ class MeshFTHIP(Function):
     """HIP-based Function for the Fourier transform of a triangle mesh."""
 
     @staticmethod
-    def forward(ctx, Verts, Faces, Faces_coeff, xi0, xi1, xi2, Filter, narrowband_thresh):
+    def forward(ctx, Verts, Faces, Faces_coeff, xi0, xi1, xi2, Filter, narrowband_thresh,
+                cache_band=True):
         if Verts.dtype != torch.float32:
             raise TypeError("Mesh FT (HIP) is implemented for float32 only, got " + str(Verts.dtype))
-        band = band_for(Filter, narrowband_thresh, xi0, xi1, xi2)
+        band = band_for(Filter, narrowband_thresh, xi0, xi1, xi2, cache=cache_band)
         V = Verts.detach().contiguous()
         Fc = Faces.contiguous()
         C = Faces_coeff.detach().to(torch.float32).contiguous()
@@ -184,7 +198,7 @@ class MeshFTHIP(Function):
         band = ctx.band
         grad_list = gather_band(grad_output, band)
         gradVerts = band_backward(V, Fc, C, grad_list, ftmesh_list, band)
-        return gradVerts, None, None, None, None, None, None, None
+        return gradVerts, None, None, None, None, None, None, None, None
 
 
 class MeshFTAutofluorescenceHIP(Function):

@@ -13,7 +13,7 @@ import numpy as np
 import torch
 from torch import nn
 
-from .hip_class import MeshFTAutofluorescenceHIP, MeshFTHIP
+from .hip_class import MeshFTAutofluorescenceHIP, MeshFTHIP, clear_band_cache
 
 
 def normalize_tensor(t, contrast_variable=0):
@@ -62,8 +62,12 @@ class Fourier3dMesh(nn.Module):
     """
 
     def __init__(self, box_size, box_shape, device='cpu', dtype=torch.float, OTF=None,
-                 narrowband_thresh=0.01):
+                 narrowband_thresh=0.01, *, cache_band=True):
         super().__init__()
+        # cache_band (not in the reference): keep the compacted band between
+        # forwards while self.OTF, its storage and version counter, the
+        # threshold and the axes stay the same; False recompacts every forward
+        self.cache_band = cache_band
         if not isinstance(box_size, torch.Tensor):
             box_size = torch.as_tensor(np.asarray(box_size))
         self.box_size = torch.tensor(box_size.clone().detach().cpu().numpy(), device=device,
@@ -86,7 +90,13 @@ class Fourier3dMesh(nn.Module):
                 "deltamic_amd computes meshFT with HIP kernels only; construct "
                 "Fourier3dMesh with a GPU device (e.g. device='cuda:0').")
         return MeshFTHIP.apply(Verts - self.box_size[:, 0], Faces, Faces_coeff, self.xi0,
-                               self.xi1, self.xi2, self.OTF, self.narrowband_thresh)
+                               self.xi1, self.xi2, self.OTF, self.narrowband_thresh,
+                               self.cache_band)
+
+    def clear_band_cache(self):
+        """Forget the band compacted from ``self.OTF`` (needed after writing the
+        OTF through ``.data`` or a raw pointer, which torch does not version)."""
+        clear_band_cache(self.OTF)
 
     def _compute_spatial_frequency_grid(self):
         return spatial_frequency_axes(self.box_size, self.box_shape, self.dtype, self.device)

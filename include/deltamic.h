@@ -227,7 +227,7 @@ int dmic_psf_aniso_grad(const float* prm, const float* grad_out, const float* ps
  *
  * No counterpart in the reference: the learned per-vertex displacement
  * proposal (15 -> 128 -> 128 -> 128 -> 3, ReLU) of the fitting loop.  Weights
- * are bf16 (packed row-major [in][out]: W1 12x128, W2 128x128, W3 128x128,
+ * are bf16 (packed row-major [in][out]: W1 15x128, W2 128x128, W3 128x128,
  * W4 128x3), biases fp32 (b1, b2, b3: 128, b4: 3); products accumulate in
  * FP32 on the matrix cores (v_mfma_f32_16x16x16_bf16).
  */

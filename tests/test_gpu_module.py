@@ -85,6 +85,34 @@ def test_band_cache_follows_otf_changes(device):
     assert b3 is not b2 and b3.idx.numel() < b2.idx.numel()
 
 
+def test_band_rebuilt_after_unversioned_otf_edit(device):
+    """A write through OTF.data bypasses torch's version counter: the module
+    recompacts after clear_band_cache(), and with cache_band=False on every
+    forward (the reference's behaviour)."""
+    from deltamic_amd import Fourier3dMesh
+    w = syn.config1()
+    mesh_ft, otf = _module(w, device)
+    V = torch.tensor(w.verts, device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.ones(len(w.faces), device=device)
+    nz0 = int((mesh_ft(V, Fc, C) != 0).sum())
+    otf.data[: w.box_shape[0] // 2] = 0.0  # unversioned edit: half the box leaves the band
+    stale = int((mesh_ft(V, Fc, C) != 0).sum())
+    assert stale == nz0  # cached band, as documented
+    mesh_ft.clear_band_cache()
+    nz1 = int((mesh_ft(V, Fc, C) != 0).sum())
+    assert 0 < nz1 < nz0
+    ref = (otf > w.thresh).sum().item()
+    assert nz1 == ref
+    fresh_otf = otf.clone()
+    nocache = Fourier3dMesh(torch.tensor(w.box_size), w.box_shape, device, torch.float32,
+                            fresh_otf, w.thresh, cache_band=False)
+    assert int((nocache(V, Fc, C) != 0).sum()) == nz1
+    fresh_otf.data.zero_()
+    fresh_otf.data[0, 0, 0] = 1.0
+    assert int((nocache(V, Fc, C) != 0).sum()) == 1
+
+
 def test_empty_band_and_empty_mesh(device):
     from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
     w = syn.config1()

@@ -286,6 +286,11 @@ def test_c_abi_rejects_bad_arguments_before_any_launch():
     assert rc == -1 and b"negative" in L.dmic_last_error()
     rc = L.dmic_psf_aniso_grad(null, null, null, null, null, null, 2, 2, 2, null, null, 0, null)
     assert rc == -1
+    # GEMM grid limits: rows (one per vertex) may be many, output columns at most 65535 tiles
+    fake = 256  # never dereferenced: the size guard returns first
+    rc = L.dmic_gemm_bf16(fake, 0, 1, 1, fake, 0, 1, 1, fake, 1, 16, 16 * 65536, 16, null, null,
+                          0, 0, null)
+    assert rc == -1 and b"65535 column tiles" in L.dmic_last_error()
     # workspaces: the autofluorescence adjoint keeps three more sums per face
     assert L.dmic_meshft_af_backward_workspace(642, 1280, 22000) > \
         L.dmic_meshft_backward_workspace(642, 1280, 22000)
