@@ -21,6 +21,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "common.h"
 #include "face_math.h"
@@ -706,14 +707,17 @@ struct FwdPlan {
   int nsplit;
 };
 
-static FwdPlan fwd_plan(int64_t nf, int64_t n) {
+// part_cap (>= 0) bounds the split partials to part_cap double2 entries
+// (nsplit * n); a band too large for two splits is evaluated unsplit.
+static FwdPlan fwd_plan(int64_t nf, int64_t n, int64_t part_cap = -1) {
   FwdPlan p{nf > 0 ? nf : 1, 1};
   const int64_t fblocks = (n + 255) / 256;
   if (fblocks <= 0 || nf <= 0) return p;
   int64_t want = (kTargetBlocks + fblocks - 1) / fblocks;
   const int64_t maxsplit = (nf + kFold - 1) / kFold;
   if (want > maxsplit) want = maxsplit;
-  if (want < 1) want = 1;
+  if (part_cap >= 0 && want > part_cap / n) want = part_cap / n;
+  if (want < 2) want = 1;
   int64_t per = (nf + want - 1) / want;
   per = (per + kFold - 1) / kFold * kFold;
   p.faces_per_split = per;
@@ -749,13 +753,13 @@ struct FwdWs {
   double2* part;
 };
 
-static FwdWs carve_fwd(Carver& c, int64_t nf, int64_t n) {
+static FwdWs carve_fwd(Carver& c, int64_t nf, int64_t n, int64_t part_cap = -1) {
   FwdWs w;
   w.flag = c.take<int>(4);
   w.cA = c.take<double>(nf > 0 ? nf : 1);
   w.S = c.take<double>(2);
   w.recs = c.take<FaceRec>(nf > 0 ? nf : 1);
-  const FwdPlan p = fwd_plan(nf, n);
+  const FwdPlan p = fwd_plan(nf, n, part_cap);
   w.part = (p.nsplit > 1) ? c.take<double2>((size_t)p.nsplit * n) : nullptr;
   return w;
 }
@@ -818,9 +822,9 @@ static int prep_faces(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, co
 template <typename IdxT, bool kAF>
 static int forward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, const float* coeff,
                         float regint, const float* xi, int64_t n, float* out, void* ws,
-                        size_t ws_bytes, hipStream_t st) {
+                        size_t ws_bytes, hipStream_t st, int64_t part_cap = -1) {
   Carver c(ws);
-  FwdWs w = carve_fwd(c, nf, n);
+  FwdWs w = carve_fwd(c, nf, n, part_cap);
   DMIC_REQUIRE(ws_bytes >= c.bytes(), DMIC_ERR_WORKSPACE,
                "forward workspace too small: %zu < %zu bytes", ws_bytes, c.bytes());
   hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, w.flag, 4, 0);
@@ -835,7 +839,7 @@ static int forward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, 
   int rc = prep_faces<IdxT>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S, w.recs,
                             st);
   if (rc != DMIC_OK) return rc;
-  const FwdPlan p = fwd_plan(nf, n);
+  const FwdPlan p = fwd_plan(nf, n, part_cap);
   dim3 grid(ceil_div(n, 256), p.nsplit);
   rc = timer_begin(0, st);
   if (rc != DMIC_OK) return rc;
@@ -1005,7 +1009,73 @@ int backward_entry(const float* verts, int64_t nv, const void* faces, int faces_
 
 }  // namespace
 
+// ------------------------------------------------------- no-allocation forward
+// dmic_meshft_forward_noalloc keeps its scratch memory (the packed face
+// records, the area sums and, when they fit, the face-split partials) in this
+// file-scope device array: one copy per GPU, held until the process ends.
+// Host code never passes `scratch` itself (its host-side name is not a device
+// address); it uses the per-device address from hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr int64_t kNoAllocMaxFaces = 262144;  // 72 B of records per face: 18.9 MB
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Address of this device's copy of `scratch`, looked up once per device with
+// that device current.
+int scratch_for_current_device(void** out) {
+  int dev = 0;
+  DMIC_HIP_CHECK(hipGetDevice(&dev));
+  DMIC_REQUIRE(dev >= 0 && dev < kMaxDevices, DMIC_ERR_ARG, "device ordinal %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    DMIC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DMIC_REQUIRE(p != nullptr, DMIC_ERR_HIP, "hipGetSymbolAddress returned null for scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return DMIC_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int dmic_meshft_forward_noalloc(const float* verts, int64_t nv, const void* faces,
+                                int faces_is_int64, int64_t nf, const float* faces_coeff,
+                                const float* xi_list, int64_t n, float* ftmesh_list,
+                                hipStream_t stream) {
+  g_err[0] = 0;
+  DMIC_REQUIRE(nv >= 0 && nf >= 0 && n >= 0, DMIC_ERR_ARG, "negative size");
+  DMIC_REQUIRE(nf <= kNoAllocMaxFaces, DMIC_ERR_ARG,
+               "nf = %lld exceeds the %lld faces the no-allocation forward's %zu-byte scratch "
+               "holds; use dmic_meshft_forward with a workspace",
+               (long long)nf, (long long)kNoAllocMaxFaces, kNoAllocScratchBytes);
+  // the face records first; the split partials only where the rest of the scratch holds them
+  Carver base(nullptr);
+  carve_fwd(base, nf, n, 0);
+  const int64_t part_cap =
+      (int64_t)((kNoAllocScratchBytes - base.bytes() - 256) / sizeof(double2));
+  DMIC_REQUIRE(n < (int64_t(1) << 40), DMIC_ERR_ARG, "size too large");
+  DMIC_REQUIRE(n == 0 || (xi_list && ftmesh_list), DMIC_ERR_ARG, "null xi_list/ftmesh_list");
+  DMIC_REQUIRE(nf == 0 || n == 0 || (verts && faces && faces_coeff), DMIC_ERR_ARG,
+               "null mesh pointer");
+  void* ws = nullptr;
+  const int rc = scratch_for_current_device(&ws);
+  if (rc != DMIC_OK) return rc;
+  if (faces_is_int64)
+    return forward_impl<int64_t, false>(verts, nv, static_cast<const int64_t*>(faces), nf,
+                                        faces_coeff, 0.0f, xi_list, n, ftmesh_list, ws,
+                                        kNoAllocScratchBytes, stream, part_cap);
+  return forward_impl<int32_t, false>(verts, nv, static_cast<const int32_t*>(faces), nf,
+                                      faces_coeff, 0.0f, xi_list, n, ftmesh_list, ws,
+                                      kNoAllocScratchBytes, stream, part_cap);
+}
 
 int dmic_meshft_forward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
                         int64_t nf, const float* faces_coeff, const float* xi_list, int64_t n,

@@ -60,6 +60,25 @@ int dmic_meshft_forward(const float* verts, int64_t nv, const void* faces, int f
                         float* ftmesh_list, void* workspace, size_t workspace_bytes,
                         hipStream_t stream);
 
+/*
+ * dmic_meshft_forward without a workspace and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Its scratch memory (packed face records
+ * and area sums; the face-split partials when they fit) is a static 32 MiB
+ * device array that each GPU the library runs on holds until the process
+ * ends.  Limit: nf <= 262144 faces (DMIC_ERR_ARG beyond); any band size n
+ * (large bands are evaluated without the face split).  Every call on one GPU
+ * shares that GPU's array, so calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller.  A face index
+ * outside [0, nv) reads as the origin and is not reported (there is no
+ * workspace for dmic_check_index_flag); validate indices beforehand.
+ */
+int dmic_meshft_forward_noalloc(const float* verts, int64_t nv, const void* faces,
+                                int faces_is_int64, int64_t nf, const float* faces_coeff,
+                                const float* xi_list, int64_t n, float* ftmesh_list,
+                                hipStream_t stream);
+
 /* Workspace bytes dmic_meshft_backward needs for these sizes. */
 size_t dmic_meshft_backward_workspace(int64_t nv, int64_t nf, int64_t n);
 

@@ -286,6 +286,11 @@ def test_c_abi_rejects_bad_arguments_before_any_launch():
     assert rc == -1 and b"negative" in L.dmic_last_error()
     rc = L.dmic_psf_aniso_grad(null, null, null, null, null, null, 2, 2, 2, null, null, 0, null)
     assert rc == -1
+    # the no-allocation forward refuses meshes beyond its static scratch before any HIP call
+    rc = L.dmic_meshft_forward_noalloc(null, 3, null, 1, 262145, null, null, 4, null, null)
+    assert rc == -1 and b"262144" in L.dmic_last_error()
+    rc = L.dmic_meshft_forward_noalloc(null, 3, null, 1, 1, null, null, 4, null, null)
+    assert rc == -1 and b"null" in L.dmic_last_error()
     # GEMM grid limits: rows (one per vertex) may be many, output columns at most 65535 tiles
     fake = 256  # never dereferenced: the size guard returns first
     rc = L.dmic_gemm_bf16(fake, 0, 1, 1, fake, 0, 1, 1, fake, 1, 16, 16 * 65536, 16, null, null,
