@@ -8,6 +8,10 @@ spherical-harmonic displacement l <= 6, c ~ N(0, 0.03), R = 1, seed 2000) in a
 on inputs already resident in HBM (vertices, faces, band frequencies, an
 upstream gradient).  value = faces * band / step time, in G face.freq/s.
 
+The metric's second half, the learned-proposal fit's wall time to plain
+Adam's loss against plain Adam's, is the `learned_fit` leg (one held-out
+shape with 5120 and one with 20480 faces, 128^3).
+
 Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs on
 one GPU.
 """
@@ -196,6 +200,51 @@ def render_leg(dev, w, band, otf, vals, reps):
     return res
 
 
+def learned_fit_leg(dev, steps):
+    """The second half of the metric: wall time of the learned-proposal fit
+    to plain Adam's loss, against plain Adam's (lr 1e-3, MSE + 0.1 uniform
+    Laplacian, 128^3, Gaussian OTF 1e-3 I, Poisson + Gaussian noise), on one
+    held-out target shape of each size (seeds 14500 at 5120 faces and 14600
+    at 20480 faces; no training fit used them).  Plain Adam runs `steps`
+    steps; its time is the wall time at which it first reached its final
+    loss; the learned fit (shipped weights, default policy) runs until it
+    reaches that loss."""
+    import torch
+    from deltamic_amd import synthetic as syn
+    from deltamic_amd.fit import fit_adam, make_problem
+    from deltamic_amd.proposal import Features, ProposalMLP, fit_learned
+    net = ProposalMLP(dev)
+    net.load_state_dict(torch.load(os.path.join(ROOT, "deltamic_amd", "weights", "proposal.pt"),
+                                   weights_only=True))
+    res = {"adam_steps": steps, "shapes": []}
+    for k, subdiv in ((10500, 4), (10600, 5)):
+        w = syn.config4(k, subdivisions=subdiv)
+        prob = make_problem(w, dev)
+        feats = Features(w.faces, prob.init_verts.shape[0], prob.lap, dev,
+                         grad_scale=net.grad_scale)
+        prob.loss_and_grad(prob.init_verts)
+        torch.cuda.synchronize()
+        adam = fit_adam(prob, steps=steps)
+        target = adam.losses[-1]
+        first = next(i for i, v in enumerate(adam.losses) if v <= target)
+        learned = fit_learned(prob, net, feats, steps=steps, stop_loss=target)
+        reached = learned.losses[-1] <= target
+        t_l = learned.times[-1]
+        rec = {"faces": len(w.faces), "seed": w.seed, "adam_final_loss": round(target, 7),
+               "adam_first_reach_step": first, "adam_s": round(adam.times[first], 3),
+               "learned_iterations": len(learned.losses), "learned_accepted": learned.accepted,
+               "learned_s": round(t_l, 3), "reached": bool(reached),
+               "speedup": round(adam.times[first] / t_l, 3)}
+        res["shapes"].append(rec)
+        log(f"learned fit, {len(w.faces)} faces (seed {w.seed}): plain Adam first reaches its "
+            f"{steps}-step loss {target:.6f} at step {first} ({adam.times[first]:.2f} s); learned "
+            f"fit {len(learned.losses)} iterations {t_l:.2f} s ({'reached' if reached else 'MISSED'}"
+            f"): {rec['speedup']:.2f}x")
+        del prob, feats
+    res["min_speedup"] = min(r["speedup"] for r in res["shapes"])
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +267,9 @@ def main() -> int:
     ap.add_argument("--render-reps", type=int, default=10,
                     help="time each op of the render chain this many times (0 = skip; "
                          "reported under 'render')")
+    ap.add_argument("--fit-steps", type=int, default=1000,
+                    help="plain-Adam steps of the learned-fit leg (0 = skip; reported under "
+                         "'learned_fit')")
     ap.add_argument("--psf-steps", type=int, default=5,
                     help="also time the learnable-PSF kernels (forward and sigma gradient) on "
                          "the box (HBM-bound; reported separately under 'psf')")
@@ -428,6 +480,10 @@ def main() -> int:
         cpu = cpu_baseline(w, band.xi_list.cpu().numpy(), Vs.numpy(), args.cpu_freqs)
         log(f"cpu baseline {cpu['value']:.4f} {UNIT} on {cpu['cores']} threads")
 
+    lfit = None
+    if args.fit_steps > 0:
+        lfit = learned_fit_leg(dev, args.fit_steps)
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "meshft_cfg2_pmc_traffic.json")
     if os.path.exists(tpath):
@@ -481,6 +537,7 @@ def main() -> int:
         },
         "full_band_evaluation": full_band,
         "autofluorescence": af,
+        "learned_fit": lfit,
         "render": render,
         "psf": psf,
         "cpu_baseline": cpu,

@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import time
 
@@ -28,7 +29,8 @@ import torch
 
 from . import synthetic as syn
 from .fit import fit_adam, make_problem
-from .proposal import Features, ProposalMLP, fit_learned, onpolicy_samples, trajectory_samples
+from .proposal import (Features, ProposalMLP, fit_learned, onpolicy_samples, trajectory_samples,
+                       trajectory_states)
 
 
 def log(*a):
@@ -72,6 +74,52 @@ def collect_onpolicy(args, net, device):
         log(f"on-policy fit {k + 1}/{args.onpolicy_fits}: {d.x.shape[0]} rows "
             f"({time.perf_counter() - t0:.1f} s)")
     return torch.cat(xs), torch.cat(ts)
+
+
+def collect_states(args, device, grad_scale):
+    """Training states for the direct objective: per training fit (seed
+    4000 + k, mesh subdivision s from --direct-fits "s:count,...") the fit
+    problem, its feature builder and the plain-Adam states every --every steps."""
+    fits = []
+    for spec in args.direct_fits.split(","):
+        subdiv, count = (int(v) for v in spec.split(":"))
+        for k in range(count):
+            t0 = time.perf_counter()
+            w = syn.config4(k, subdivisions=subdiv, box=args.box)
+            prob = make_problem(w, device)
+            feats = Features(w.faces, len(w.extra["init_verts"]), prob.lap, device,
+                             grad_scale=grad_scale)
+            states = trajectory_states(prob, args.train_steps, args.every)
+            fits.append((prob, feats, states, len(w.faces)))
+            log(f"states: fit {k + 1}/{count} ({len(w.faces)} faces): {len(states)} states "
+                f"({time.perf_counter() - t0:.1f} s)")
+    return fits
+
+
+def train_direct(net, fits, epochs, lr, seed, passes):
+    """Descends the mean log L(V + d) over all states (one state per network
+    step, shuffled each epoch); logs the mean log-ratio L(V + d) / L(V) per
+    face count, the fraction by which one proposal lowers the loss."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    items = [(i, j) for i, (_, _, st, _) in enumerate(fits) for j in range(len(st))]
+    hist = []
+    for ep in range(epochs):
+        perm = torch.randperm(len(items), generator=g).tolist()
+        acc = {}
+        t0 = time.perf_counter()
+        for p in perm:
+            i, j = items[p]
+            prob, feats, states, nf = fits[i]
+            V, gr, adir, L0 = states[j]
+            Lc = net.direct_step(prob, feats, V, gr, adir, lr=lr, passes=passes)
+            a = acc.setdefault(nf, [0.0, 0])
+            a[0] += math.log(max(Lc, 1e-30) / max(L0, 1e-30))
+            a[1] += 1
+        rec = {str(nf): a[0] / a[1] for nf, a in acc.items()}
+        hist.append(rec)
+        log(f"direct epoch {ep + 1}/{epochs} ({time.perf_counter() - t0:.1f} s): mean log "
+            f"L(V+d)/L(V) " + ", ".join(f"{k} faces {v:+.4f}" for k, v in rec.items()))
+    return hist
 
 
 def train(net, x, t, epochs, batch, lr, seed):
@@ -119,7 +167,10 @@ def evaluate(net, args, device, subdiv, k):
         t_pre = time.perf_counter() - t0
     learned = fit_learned(prob, net, feats, steps=args.eval_steps, stop_loss=target, precond=precond,
                           patience=args.patience, cooldown=args.cooldown,
-                          cooldown_max=args.cooldown_max, accept_margin=args.accept_margin)
+                          cooldown_max=args.cooldown_max, accept_margin=args.accept_margin,
+                          line_search=args.line_search, accept_rule=args.accept_rule,
+                          scale_max=args.scale_max,
+                          smooth_passes=args.smooth_passes if args.smooth_passes > 0 else None)
     reached = bool(learned.losses and learned.losses[-1] <= target)
     t_learned = (learned.times[-1] + t_pre) if learned.times else float("nan")
     rec = {
@@ -194,6 +245,22 @@ def main(argv=None):
                          "decrease of the Adam step it replaces")
     ap.add_argument("--cooldown-max", type=int, default=256,
                     help="pauses double up to this length")
+    ap.add_argument("--direct-fits", default="",
+                    help="train through the meshFT layer (descend log L(V + d)) on plain-Adam "
+                         "states of these fits, 'subdivision:count,...' (e.g. '4:24,5:8'), "
+                         "starting from --load's weights or a fresh network")
+    ap.add_argument("--direct-epochs", type=int, default=10)
+    ap.add_argument("--direct-lr", type=float, default=1e-4)
+    ap.add_argument("--accept-rule", choices=("adam", "margin"), default="adam",
+                    help="adam: accept a proposal only if it beats the plain Adam step's actual "
+                         "loss (and the current loss); margin: beat --accept-margin times the "
+                         "Adam step's first-order decrease")
+    ap.add_argument("--scale-max", type=float, default=1.0,
+                    help="largest trust scale applied to the network's proposal")
+    ap.add_argument("--line-search", type=int, default=0,
+                    help="fit_learned: doublings of an accepted proposal's scale while they help")
+    ap.add_argument("--smooth-passes", type=int, default=2,
+                    help="1-ring smoothing passes of each proposal (0: scaled with the mesh)")
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
     device = torch.device("cuda:0")
@@ -201,7 +268,24 @@ def main(argv=None):
     summary = {"args": vars(args)}
     if args.load:
         net.load_state_dict(torch.load(args.load, weights_only=True))
-    else:
+    if args.direct_fits:
+        if net.grad_scale is None:
+            raise SystemExit("--direct-fits needs --load weights (their gradient feature scale)")
+        t0 = time.perf_counter()
+        fits = collect_states(args, device, net.grad_scale)
+        summary["direct_states"] = {str(nf): sum(len(f[2]) for f in fits if f[3] == nf)
+                                    for nf in sorted({f[3] for f in fits})}
+        summary["direct_collect_s"] = time.perf_counter() - t0
+        net.step_count = 0  # fresh Adam moments for the network's own optimiser
+        net.mw.zero_(); net.vw.zero_(); net.mb.zero_(); net.vb.zero_()
+        t1 = time.perf_counter()
+        passes = args.smooth_passes if args.smooth_passes > 0 else 2
+        summary["direct_train"] = train_direct(net, fits, args.direct_epochs, args.direct_lr,
+                                               seed=3, passes=passes)
+        summary["direct_train_s"] = time.perf_counter() - t1
+        torch.save(net.state_dict(), os.path.join(args.out, "proposal.pt"))
+        del fits
+    elif not args.load:
         t0 = time.perf_counter()
         x, t, gs = collect(args, device)
         net.grad_scale = gs

@@ -12,14 +12,14 @@ per-vertex features
     step m_hat / (sqrt(v_hat) + eps) that Adam would take next
 
 to a displacement, which is smoothed over the 1-ring before use.  The
-network is trained on the method's own plain-Adam fitting trajectories: the
-target for a snapshot (V_k, grad_k, Adam state_k) is the displacement Adam
-makes over the next ``horizon`` steps.  In the fit, a proposal V + d is
-accepted only if the meshFT + OTF loss decreases by more than 8 times the
-first-order decrease of the Adam step it replaces; otherwise a plain Adam
-step is taken and the proposal's trust scale halves (fit_learned).  A
-margin of 1 accepts proposals that barely beat an Adam step and whose
-jumps the following Adam steps partly undo; 8 keeps only large gains.
+network is trained on the method's own fitting trajectories in two ways:
+by regression on the displacement plain Adam makes over the next
+``horizon`` steps (train_step), and through the meshFT layer itself
+(direct_step): the proposal is applied to a trajectory state and the
+network descends log L(V + d), with dL/dd from the HIP meshFT adjoint.  In
+the fit (fit_learned) each proposal competes with the plain Adam step it
+would replace: it is accepted only if its meshFT + OTF loss is below both
+the current loss and the Adam step's loss, otherwise the Adam step stands.
 """
 from __future__ import annotations
 
@@ -88,6 +88,16 @@ class Features:
         for _ in range(passes):
             d = 0.5 * (d + (d[lap.nbr] * lap.mask[..., None]).sum(1) * lap.invdeg[:, None])
         return d
+
+    @torch.no_grad()
+    def smooth_adjoint(self, g: torch.Tensor, passes: int = 2) -> torch.Tensor:
+        """Transpose of smooth: with W the row-normalised adjacency, a pass is
+        (I + W)/2 and its transpose (I + W^T)/2, (W^T g)_u = sum_{v in N(u)} g_v / deg v."""
+        lap = self.lap
+        for _ in range(passes):
+            gs = g * lap.invdeg[:, None]
+            g = 0.5 * (g + (gs[lap.nbr] * lap.mask[..., None]).sum(1))
+        return g
 
 
 class ProposalMLP:
@@ -189,11 +199,38 @@ class ProposalMLP:
         return loss
 
 
+    def direct_step(self, problem: FitProblem, feats: Features, V: torch.Tensor,
+                    g: torch.Tensor, adir: torch.Tensor, lr: float = 1e-4,
+                    passes: int = 2) -> float:
+        """One training step through the meshFT layer: the proposal
+        d = smooth(out_scale net(x)) at state (V, grad, Adam direction) is
+        applied, and the network descends log L(V + d), the fit loss after
+        the jump.  dL/dd comes from the meshFT adjoint (one forward and one
+        adjoint of the HIP kernels at the candidate), then through the
+        smoothing's transpose and the MLP backward.  Returns L(V + d)."""
+        x = feats(V, g, adir)
+        y, acts = self.forward(x, keep_acts=True)
+        d = feats.smooth(self.out_scale * y, passes)
+        Lc, gc = problem.loss_and_grad(V + d)
+        Lc_f = float(Lc)
+        dy = feats.smooth_adjoint(gc, passes) * (self.out_scale / max(Lc_f, 1e-30))
+        gw, gb = self.backward(x, acts, dy)
+        self.adam(gw, gb, lr=lr)
+        return Lc_f
+
+
 # ------------------------------------------------------------------ training data
 @dataclass
 class TrajectoryData:
     x: torch.Tensor        # [rows, 15] features
     target: torch.Tensor   # [rows, 3] displacement over the horizon
+
+
+def trajectory_states(problem: FitProblem, steps: int, every: int, lr: float = 1e-3):
+    """(vertices, gradient, Adam direction, loss) every ``every`` steps of a plain Adam fit."""
+    from .fit import fit_adam
+    res = fit_adam(problem, steps=steps, lr=lr, snapshot_every=every, sync_every=0)
+    return [(V, g, adir, res.losses[k]) for k, V, g, adir in res.snapshots]
 
 
 def trajectory_samples(problem: FitProblem, feats: Features, steps: int, every: int,
@@ -240,10 +277,12 @@ def _adam_absorb(opt: torch.optim.Adam, p: torch.Tensor, g: torch.Tensor) -> Non
 
 def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: int = 1000,
                 lr: float = 1e-3, stop_loss: float | None = None, scale0: float = 1.0,
-                grow: float = 1.5, shrink: float = 0.5, scale_max: float = 8.0,
+                grow: float = 1.5, shrink: float = 0.5, scale_max: float = 1.0,
                 patience: int = 4, cooldown: int = 8, cooldown_max: int = 256,
                 snapshot_every: int = 0, accept_margin: float = 8.0,
-                precond=None) -> FitResult:
+                precond=None, smooth_passes: int | None = 2,
+                line_search: int = 0, absorb: bool = True,
+                accept_rule: str = "adam") -> FitResult:
     """Accept/reject fit.  Each iteration the network proposes V + s*d (d
     smoothed over the 1-ring; trust scale s grows after an accepted proposal,
     shrinks after a rejection).  The candidate's loss is computed with its
@@ -256,7 +295,18 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
     plain Adam step is taken.  After ``patience`` rejections in a row no
     proposal is made for ``cooldown`` iterations, a pause that doubles (up to
     ``cooldown_max``) each time it recurs without an acceptance in between;
-    the trust scale restarts from ``scale0`` after each pause.  With
+    the trust scale restarts from ``scale0`` after each pause.
+    ``smooth_passes`` 1-ring averaging passes are applied to each proposal
+    (None: scaled with the mesh, 2 per 2562 vertices, so the smoothing
+    length stays the same on finer meshes).  With ``line_search`` > 0 a
+    candidate that passes the bar is tried again at twice its scale (one
+    more forward each, at most ``line_search`` times) while that lowers the
+    loss.  ``absorb=False`` leaves Adam's moments as they were when a
+    proposal is accepted.  ``accept_rule="adam"`` replaces the margin test
+    by the exact one: the plain Adam step is taken first (one more forward
+    per proposal) and the proposal is accepted only if its loss is below
+    both the current loss and the Adam step's loss; otherwise the Adam step
+    stands and its graph gives the next gradient.  With
     ``precond`` (an optim.LargeStepsPreconditioner) the smoothed proposal is
     further multiplied by (I + lambda L)^-2, which keeps its smooth
     components and damps the rest.
@@ -264,6 +314,8 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
     (k, vertices, gradient, Adam direction, Adam state) is kept every that many
     iterations (on-policy training data, onpolicy_samples)."""
     Vp = problem.init_verts.clone().requires_grad_(True)
+    if smooth_passes is None:
+        smooth_passes = max(2, round(2 * Vp.shape[0] / 2562))
     opt = torch.optim.Adam([Vp], lr=lr)
     losses, times = [], []
     accepted = proposals = 0
@@ -287,16 +339,42 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
             quiet -= 1
         else:
             adir = adam_direction(opt, Vp, g)
-            d = feats.smooth(net.propose(feats(Vp.detach(), g, adir)))
+            d = feats.smooth(net.propose(feats(Vp.detach(), g, adir)), passes=smooth_passes)
             if precond is not None:
                 d = precond.solve(d)
             proposals += 1
-            cand = (Vp.detach() + s * d).requires_grad_(True)
+            V0 = Vp.detach().clone()
+            La = None
+            if accept_rule == "adam":
+                # the plain Adam step, taken for real: its loss is part of the
+                # bar and its graph gives the gradient if it wins; opt.step()
+                # also advances Adam's moments with g (what absorbing does)
+                Vp.grad = g
+                opt.step()
+                La = problem.loss(Vp)
+                bar = min(float(L), float(La.detach()))
+            else:
+                bar = float(L) + accept_margin * _adam_predicted_decrease(opt, Vp, g, adir)
+            # the candidate at the trust scale; while it passes the bar and
+            # improves, up to line_search more at twice the previous scale
+            cand = (V0 + s * d).requires_grad_(True)
             Lc = problem.loss(cand)
-            bar = float(L) + accept_margin * _adam_predicted_decrease(opt, Vp, g, adir)
+            sc = s
+            for _ in range(line_search):
+                if float(Lc.detach()) >= bar or sc >= scale_max:
+                    break
+                sc2 = min(2.0 * sc, scale_max)
+                c2 = (V0 + sc2 * d).requires_grad_(True)
+                l2 = problem.loss(c2)
+                if float(l2.detach()) >= float(Lc.detach()):
+                    break
+                cand, Lc, sc = c2, l2, sc2
+            s = sc
             if float(Lc.detach()) < bar:
                 (gc,) = torch.autograd.grad(Lc, cand)
-                _adam_absorb(opt, Vp, g)
+                if La is None and absorb:
+                    _adam_absorb(opt, Vp, g)
+                del La
                 with torch.no_grad():
                     Vp.copy_(cand.detach())
                 L, g = Lc.detach(), gc
@@ -314,6 +392,10 @@ def fit_learned(problem: FitProblem, net: ProposalMLP, feats: Features, steps: i
                     quiet, misses = pause, 0
                     pause = min(2 * pause, cooldown_max)
                     s = scale0  # the next proposal starts again from the full step
+                if La is not None:  # the Adam step already taken wins
+                    (g,) = torch.autograd.grad(La, Vp)
+                    L = La.detach()
+                    took = True
         if not took:
             Vp.grad = g
             opt.step()
