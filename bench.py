@@ -200,6 +200,55 @@ def render_leg(dev, w, band, otf, vals, reps):
     return res
 
 
+def closure_leg(dev, w, Vs, reps):
+    """The reference fit loop's OTF (Gradient_Descent.ipynb cell 2): OTF =
+    |fftshift(fftn(PSF))| of Gaussian_psf(box_shape, box_size[:, 1], sigma=1e3),
+    band threshold 1e-2, on the bench mesh and box.  That band is not
+    centrosymmetric (fftshift centres it on k = n/2, the grid mirror is
+    k <-> n-1-k), so it is evaluated through its symmetric closure; timed
+    against evaluating every band frequency (forward + adjoint each)."""
+    import torch
+    from deltamic_amd import hip_class
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    from deltamic_amd.psf import Gaussian_psf
+    from deltamic_amd.render import spatial_frequency_axes
+    bs = torch.tensor(w.box_size, dtype=torch.float32, device=dev)
+    axes = spatial_frequency_axes(bs, w.box_shape, torch.float32, dev)
+    psf = Gaussian_psf(w.box_shape, torch.tensor(w.box_size)[:, 1], sigma=1e3, device=dev)
+    with torch.no_grad():
+        otf = torch.abs(torch.fft.fftshift(torch.fft.fftn(psf.forward()))).contiguous()
+    band = hip_class.compute_band(otf, 1e-2, *axes)
+    V = Vs.to(dev)
+    Fc = torch.tensor(w.faces, device=dev)
+    C = torch.tensor(w.coeff, device=dev)
+    g = torch.randn(band.n, dtype=torch.complex64, device=dev)
+
+    def full():
+        ft = forward_fourier3d_cpp(V, Fc, C, band.xi_list)
+        backward_fourier3d_cpp(V, Fc, C, g, ft, band.xi_list)
+
+    def closed():
+        ft = hip_class.band_forward(V, Fc, C, band)
+        hip_class.band_backward(V, Fc, C, g, ft, band)
+
+    res = {"band": band.n, "symmetric": band.symmetric,
+           "closure": band.closure.n if band.closure is not None else None}
+    for name, fn in (("full_band", full), ("closure_halving", closed)):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / reps
+        res[name] = {"ms_per_step": round(ms, 4),
+                     "value": round(len(w.faces) * band.n / (ms * 1e-3) / 1e9, 3)}
+    return res
+
+
 def learned_fit_leg(dev, steps):
     """The second half of the metric: wall time of the learned-proposal fit
     to plain Adam's loss, against plain Adam's (lr 1e-3, MSE + 0.1 uniform
@@ -267,6 +316,9 @@ def main() -> int:
     ap.add_argument("--render-reps", type=int, default=10,
                     help="time each op of the render chain this many times (0 = skip; "
                          "reported under 'render')")
+    ap.add_argument("--closure-steps", type=int, default=3,
+                    help="time the fit loop's |fftshift(fftn(PSF))| OTF band through its "
+                         "symmetric closure and in full (0 = skip; reported under 'fftshift_otf')")
     ap.add_argument("--fit-steps", type=int, default=1000,
                     help="plain-Adam steps of the learned-fit leg (0 = skip; reported under "
                          "'learned_fit')")
@@ -480,6 +532,13 @@ def main() -> int:
         cpu = cpu_baseline(w, band.xi_list.cpu().numpy(), Vs.numpy(), args.cpu_freqs)
         log(f"cpu baseline {cpu['value']:.4f} {UNIT} on {cpu['cores']} threads")
 
+    closure = None
+    if args.closure_steps > 0:
+        closure = closure_leg(dev, w, Vs, args.closure_steps)
+        log(f"fftshift OTF band {closure['band']} (closure {closure['closure']}): full band "
+            f"{closure['full_band']['value']} {UNIT}, through the closure "
+            f"{closure['closure_halving']['value']} {UNIT}")
+
     lfit = None
     if args.fit_steps > 0:
         lfit = learned_fit_leg(dev, args.fit_steps)
@@ -537,6 +596,7 @@ def main() -> int:
         },
         "full_band_evaluation": full_band,
         "autofluorescence": af,
+        "fftshift_otf": closure,
         "learned_fit": lfit,
         "render": render,
         "psf": psf,

@@ -20,7 +20,13 @@ work done by HIP kernels:
   entry n-1-j is the mirror -xi of entry j and ftmesh(-xi) = conj(ftmesh(xi)),
   so the forward evaluates the first (n+1)/2 frequencies and mirrors the rest,
   and the adjoint runs on the same half with the folded upstream gradient
-  g[j] + conj(g[n-1-j]).  ``HERMITIAN_HALVING = False`` turns it off.
+  g[j] + conj(g[n-1-j]).  A band that is not centrosymmetric on an exactly
+  antisymmetric grid (e.g. the reference fit loop's |fftshift(fftn(PSF))|
+  OTF on an even box, symmetric under k <-> n-k rather than n-1-k) is
+  evaluated through its symmetric closure B u -B: half of the closure is
+  evaluated, mirrored, and B's entries are gathered from it (the adjoint
+  scatters B's upstream gradient into the closure first).
+  ``HERMITIAN_HALVING = False`` turns both off.
 
 Unlike cuda_class.py:8 importing this module does not reseed torch's RNG.
 """
@@ -45,6 +51,8 @@ class Band:
     shape: tuple
     key: tuple
     symmetric: bool = False  # entry n-1-j is exactly -xi of entry j
+    closure: "Band | None" = None   # symmetric closure B u -B of a non-centrosymmetric band
+    pos: torch.Tensor | None = None  # int32 [n]: position of each entry in the closure
 
     @property
     def n(self) -> int:
@@ -62,8 +70,10 @@ def _thresh_value(narrowband_thresh) -> float:
 
 
 def compute_band(Filter: torch.Tensor, thresh: float, xi0: torch.Tensor, xi1: torch.Tensor,
-                 xi2: torch.Tensor) -> Band:
-    """Compacts the band ``Filter > thresh`` (HIP kernels; one host sync for the count)."""
+                 xi2: torch.Tensor, closure: bool = True) -> Band:
+    """Compacts the band ``Filter > thresh`` (HIP kernels; host syncs for the
+    count and the symmetry flag).  With ``closure``, a band that is not
+    centrosymmetric on antisymmetric axes also gets its symmetric closure."""
     if not Filter.is_cuda:
         raise RuntimeError("Filter must be a CUDA tensor")
     if Filter.dim() != 3:
@@ -98,7 +108,17 @@ def compute_band(Filter: torch.Tensor, thresh: float, xi0: torch.Tensor, xi1: to
         _lib.check(L.dmic_band_symmetry(f.data_ptr(), n0, n1, n2, float(thresh),
                                         axes[0].data_ptr(), axes[1].data_ptr(), axes[2].data_ptr(),
                                         flag.data_ptr(), st), "dmic_band_symmetry")
-        symmetric = int(flag.item()) == 0
+        flag_v = int(flag.item())
+        symmetric = flag_v == 0
+        if flag_v == 1 and closure:
+            # mask not centrosymmetric on antisymmetric axes: the closure is the
+            # band of max(Filter, Filter mirrored), mirror = flip of all three axes
+            fsym = torch.maximum(f, torch.flip(f, dims=(0, 1, 2))).contiguous()
+            cb = compute_band(fsym, thresh, xi0, xi1, xi2, closure=False)
+            if cb.symmetric:
+                pos = torch.searchsorted(cb.idx, idx, out_int32=True)
+                return Band(idx=idx, xi_list=xi_list, shape=(n0, n1, n2), key=(),
+                            symmetric=False, closure=cb, pos=pos)
     return Band(idx=idx, xi_list=xi_list, shape=(n0, n1, n2), key=(), symmetric=symmetric)
 
 
@@ -147,8 +167,16 @@ def gather_band(box: torch.Tensor, band: Band) -> torch.Tensor:
 
 
 def band_forward(V, Fc, C, band: Band, regint=None) -> torch.Tensor:
-    """ftmesh_list [n] complex64 on the band (half band + mirror when symmetric;
-    both variants are Hermitian: real face weights and xi . nor2 odd in xi)."""
+    """ftmesh_list [n] complex64 on the band (half band + mirror when symmetric,
+    gathered from the half-evaluated closure when the band has one; both
+    variants are Hermitian: real face weights and xi . nor2 odd in xi)."""
+    if HERMITIAN_HALVING and band.closure is not None:
+        vals_c = band_forward(V, Fc, C, band.closure, regint=regint)
+        out = torch.empty((band.n,), dtype=torch.complex64, device=V.device)
+        L = _lib.lib()
+        _lib.check(L.dmic_band_gather(vals_c.data_ptr(), band.pos.data_ptr(), band.n,
+                                      out.data_ptr(), _lib.stream_of(out)), "dmic_band_gather")
+        return out
     if not (HERMITIAN_HALVING and band.symmetric and band.n > 1):
         return forward_fourier3d_cpp(V, Fc, C, band.xi_list, regint=regint)
     out = torch.empty((band.n,), dtype=torch.complex64, device=V.device)
@@ -160,7 +188,18 @@ def band_forward(V, Fc, C, band: Band, regint=None) -> torch.Tensor:
 
 
 def band_backward(V, Fc, C, grad_list, ftmesh_list, band: Band, regint=None) -> torch.Tensor:
-    """gradVerts [nv, 3] from the band upstream gradient (folded to the half band when symmetric)."""
+    """gradVerts [nv, 3] from the band upstream gradient (folded to the half band when symmetric;
+    scattered into the closure first when the band has one)."""
+    if HERMITIAN_HALVING and band.closure is not None:
+        cb = band.closure
+        g_c = torch.empty((cb.n,), dtype=torch.complex64, device=V.device)
+        L = _lib.lib()
+        _lib.check(L.dmic_band_scatter(grad_list.data_ptr(), band.pos.data_ptr(), band.n,
+                                       g_c.data_ptr(), cb.n, _lib.stream_of(g_c)),
+                   "dmic_band_scatter")
+        # the adjoint does not read ftmesh_list (it rebuilds the area term from
+        # its own per-face sums), so the closure gradient stands in for it
+        return band_backward(V, Fc, C, g_c, g_c, cb, regint=regint)
     if not (HERMITIAN_HALVING and band.symmetric and band.n > 1):
         return backward_fourier3d_cpp(V, Fc, C, grad_list, ftmesh_list, band.xi_list,
                                       regint=regint)

@@ -312,6 +312,43 @@ def test_asymmetric_band_is_not_halved(device):
     assert not b2.symmetric
 
 
+def test_closure_halving_on_fftshift_otf(device):
+    """The reference fit loop's OTF, |fftshift(fftn(PSF))| of a learnable
+    Gaussian PSF (Gradient_Descent.ipynb cell 2, threshold 1e-2), is symmetric
+    under k <-> n-k, not under the grid mirror k <-> n-1-k, so its band is not
+    centrosymmetric.  Forward and adjoint through the half-evaluated symmetric
+    closure equal the full-band evaluation (rel-L2 <= 1e-6)."""
+    from deltamic_amd import hip_class
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    from deltamic_amd.psf import Gaussian_psf
+    w = syn.config1()
+    bs, axes, _, _ = _band(w, device)
+    psf = Gaussian_psf(w.box_shape, torch.tensor(w.box_size)[:, 1], sigma=1e3, device=device)
+    with torch.no_grad():
+        otf = torch.abs(torch.fft.fftshift(torch.fft.fftn(psf.forward()))).contiguous()
+    band = hip_class.compute_band(otf, 1e-2, *axes)
+    assert not band.symmetric and band.closure is not None and band.closure.symmetric
+    cb = band.closure
+    print(f"fftshift OTF band {band.n}, closure {cb.n} ({cb.n / band.n:.3f}x), evaluated "
+          f"{cb.half}")
+    assert band.n < cb.n < 1.3 * band.n
+    # the closure holds every band entry at pos, with the same frequency
+    assert torch.equal(cb.idx[band.pos.long()], band.idx)
+    assert torch.equal(cb.xi_list[band.pos.long()], band.xi_list)
+    V = torch.tensor(_shifted(w, bs), device=device)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.tensor(w.coeff, device=device)
+    full = forward_fourier3d_cpp(V, Fc, C, band.xi_list)
+    half = hip_class.band_forward(V, Fc, C, band)
+    ef = O.rel_l2(half.cpu().numpy(), full.cpu().numpy())
+    g = torch.tensor(_grad(band.n, 78), device=device)
+    gfull = backward_fourier3d_cpp(V, Fc, C, g, full, band.xi_list)
+    ghalf = hip_class.band_backward(V, Fc, C, g, half, band)
+    eb = O.rel_l2(ghalf.cpu().numpy(), gfull.cpu().numpy())
+    print(f"closure vs full band: forward rel-L2 {ef:.2e}, adjoint rel-L2 {eb:.2e}")
+    assert ef <= 1e-6 and eb <= 1e-6
+
+
 def test_full_size_cfg2_properties(device):
     """Config 2 at full size (all 1.08M band frequencies, 20,480 faces), checked
     through properties that need no oracle: Hermitian symmetry of the
