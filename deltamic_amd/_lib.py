@@ -57,6 +57,9 @@ SIGNATURES = {
     "dmic_psf_aniso": ([_vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp], _int),
     "dmic_psf_aniso_grad": ([_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _sz,
                              _vp], _int),
+    "dmic_laplacian_solve_workspace": ([_i64], _sz),
+    "dmic_laplacian_solve": ([_vp, _vp, _i64, _f32, _int, _vp, _vp, _f32, _int, _vp, _vp, _sz,
+                              _vp], _int),
     "dmic_gemm_bf16": ([_vp, _int, _i64, _i64, _vp, _int, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
                         _vp, _vp, _i64, _int, _vp], _int),
     "dmic_mlp_weight_count": ([], _i64),

@@ -242,6 +242,24 @@ int dmic_psf_aniso_grad(const float* prm, const float* grad_out, const float* ps
                         int64_t n2, float* grad_prm, void* workspace, size_t workspace_bytes,
                         hipStream_t stream);
 
+/* ------------------------------------------------- large-steps preconditioner
+ *
+ * x = (I + lambda L)^-power g for the vertex gradient g (float [nv][3]),
+ * L = D - A the combinatorial Laplacian of the mesh's vertex graph given as
+ * CSR adjacency (row_ptr[nv + 1], col[row_ptr[nv]]: each vertex's
+ * neighbours, no self loops; device int32).  Replaces the reference fit
+ * loop's sparse Cholesky solve of (I + lambda L)^2 (Gradient_Descent.ipynb
+ * cell 2: compute_matrix, CholeskySolver(M @ M), solver.solve) with `power`
+ * Jacobi-preconditioned conjugate-gradient solves of M = I + lambda L to a
+ * relative residual `tol` (at most max_iter iterations each) in one
+ * workgroup; FP64 vectors, fixed-order reductions (bitwise reproducible).
+ * iters_out (optional, device int) receives the last solve's iterations.
+ */
+size_t dmic_laplacian_solve_workspace(int64_t nv);
+int dmic_laplacian_solve(const int* row_ptr, const int* col, int64_t nv, float lambda, int power,
+                         const float* g, float* x, float tol, int max_iter, int* iters_out,
+                         void* workspace, size_t workspace_bytes, hipStream_t stream);
+
 /* ------------------------------------------------- proposal network (MLP)
  *
  * No counterpart in the reference: the learned per-vertex displacement

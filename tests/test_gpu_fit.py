@@ -115,6 +115,30 @@ def test_trained_proposal_gains_over_plain_adam(device):
     assert learned.times[-1] < 1.1 * adam.times[first]
 
 
+def test_sparse_large_steps_solve_20480_faces(device):
+    """(I + 50 L)^-2 g on the 10242-vertex icosphere with the sparse
+    conjugate-gradient kernel against scipy's sparse LU solve of M^2 in
+    FP64 (rel-L2 <= 1e-5), with constants as fixed points."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    from deltamic_amd.optim import LargeStepsPreconditioner, vertex_adjacency_csr
+    u, f = syn.icosphere(5)
+    nv = len(u)
+    row, col = vertex_adjacency_csr(f, nv)
+    A = sp.csr_matrix((np.ones(len(col)), col, row), shape=(nv, nv))
+    M = (sp.identity(nv) + 50.0 * (sp.diags(np.asarray(A.sum(1)).ravel()) - A)).tocsc()
+    g = syn.rng(4004).normal(size=(nv, 3))
+    x64 = spl.splu((M @ M).tocsc()).solve(g)
+    pre = LargeStepsPreconditioner(f, nv, 50.0, device)
+    x = pre.solve(torch.tensor(g, dtype=torch.float32, device=device)).cpu().double().numpy()
+    err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    c = pre.solve(torch.ones((nv, 3), device=device))
+    print(f"sparse large-steps solve, {nv} vertices: rel-L2 {err:.2e}, "
+          f"{int(pre.iters.item())} CG iterations in the second solve")
+    assert err <= 1e-5
+    assert torch.allclose(c, torch.ones_like(c), atol=1e-6)
+
+
 def test_learned_fit_three_times_faster_at_20480_faces(device):
     """Held-out 20480-face shape (icosphere s5 radially displaced by SH l <= 8,
     target seed 14600; never a training fit): the shipped network with the
@@ -161,9 +185,13 @@ def test_large_steps_fit_decreases_loss(device):
     x64 = np.linalg.solve(M @ M, g.cpu().double().numpy())
     err = np.linalg.norm(pre.solve(g).cpu().double().numpy() - x64) / np.linalg.norm(x64)
     assert err < 1e-5
+    dense = LargeStepsPreconditioner(w.faces, nv, 50.0, device, method="dense")
+    err_d = np.linalg.norm(dense.solve(g).cpu().double().numpy() - x64) / np.linalg.norm(x64)
+    assert err_d < 1e-5
     res = fit_largesteps(prob, steps=40, lr=1e-2, lambda_=50.0)
     print(f"large-steps fit: loss {res.losses[0]:.5f} -> {res.losses[-1]:.5f} in 40 steps "
-          f"({res.times[-1]:.2f} s); preconditioner error {err:.1e}")
+          f"({res.times[-1]:.2f} s); preconditioner error sparse {err:.1e} "
+          f"({int(pre.iters.item())} CG iterations in the second solve), dense {err_d:.1e}")
     # AdamUniform moves every vertex by at most lr per step: a steady, modest decrease
     assert res.losses[-1] < 0.9 * res.losses[0]
     assert np.mean(res.losses[20:]) < np.mean(res.losses[:20])

@@ -49,7 +49,7 @@ def test_large_steps_matrix_and_preconditioner():
     assert set(np.unique(deg)) == {5.0, 6.0}  # icosphere valences
     M = compute_matrix(f, nv, 50.0, "cpu").numpy()
     assert np.allclose(M, np.eye(nv) + 50.0 * L)
-    pre = LargeStepsPreconditioner(f, nv, 50.0, "cpu")
+    pre = LargeStepsPreconditioner(f, nv, 50.0, "cpu", method="dense")
     g = torch.tensor(syn.rng(22).normal(size=(nv, 3)), dtype=torch.float32)
     x = pre.solve(g).double().numpy()
     # FP32 storage of the dense inverse: compare with the FP64 solve of M^2 x = g
@@ -58,3 +58,17 @@ def test_large_steps_matrix_and_preconditioner():
     # constants are fixed points (L annihilates them)
     c = torch.ones((nv, 3))
     assert torch.allclose(pre.solve(c), c, atol=1e-5)
+
+
+def test_vertex_adjacency_csr_matches_laplacian():
+    from deltamic_amd.optim import vertex_adjacency_csr
+    u, f = syn.icosphere(2)
+    nv = len(u)
+    row, col = vertex_adjacency_csr(f, nv)
+    L = combinatorial_laplacian(f, nv, "cpu").numpy()
+    A = np.zeros((nv, nv))
+    for i in range(nv):
+        nb = col[row[i]:row[i + 1]]
+        assert np.all(np.diff(nb) > 0) and i not in nb
+        A[i, nb] = 1.0
+    assert np.array_equal(np.diag(L), A.sum(1)) and np.array_equal(L, np.diag(A.sum(1)) - A)
