@@ -28,6 +28,10 @@ SIGNATURES = {
     "dmic_meshft_forward_workspace": ([_i64, _i64, _i64], _sz),
     "dmic_meshft_forward": (
         [_vp, _i64, _vp, _int, _i64, _vp, _vp, _i64, _vp, _vp, _sz, _vp], _int),
+    "dmic_meshft_forward_f64": (
+        [_vp, _i64, _vp, _int, _i64, _vp, _vp, _i64, _vp, _vp, _sz, _vp], _int),
+    "dmic_meshft_backward_f64": (
+        [_vp, _i64, _vp, _int, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _sz, _vp], _int),
     "dmic_meshft_forward_noalloc": ([_vp, _i64, _vp, _int, _i64, _vp, _vp, _i64, _vp, _vp], _int),
     "dmic_meshft_backward_workspace": ([_i64, _i64, _i64], _sz),
     "dmic_meshft_backward": (

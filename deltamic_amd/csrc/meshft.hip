@@ -70,17 +70,18 @@ constexpr int kTargetBlocks = DMIC_TARGET_BLOCKS;
 
 // -------------------------------------------------------------------- prep
 
-template <typename IdxT>
-__device__ __forceinline__ void load_face_verts(const float* __restrict__ V, int64_t nv,
+// Vertices are float or double (T); the corners come back in FP64 (exact).
+template <typename IdxT, typename T>
+__device__ __forceinline__ void load_face_verts(const T* __restrict__ V, int64_t nv,
                                                 const IdxT* __restrict__ Fc, int64_t t,
-                                                float (&P)[3][3], int* flag) {
+                                                double (&P)[3][3], int* flag) {
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const int64_t i = (int64_t)Fc[3 * t + j];
     if (i >= 0 && i < nv) {
       P[j][0] = V[3 * i]; P[j][1] = V[3 * i + 1]; P[j][2] = V[3 * i + 2];
     } else {
-      P[j][0] = P[j][1] = P[j][2] = 0.0f;
+      P[j][0] = P[j][1] = P[j][2] = 0.0;
       flag[0] = 1;
     }
   }
@@ -90,16 +91,16 @@ __device__ __forceinline__ void load_face_verts(const float* __restrict__ V, int
 // workgroup and a fixed order, S[0] = sum_t c_t A_t and
 // S[1] = regvol = -sum_t a . (b x c) / 6 (compute_volume_manifold,
 // render.py:221-226; used by the autofluorescence variant).
-template <typename IdxT>
+template <typename IdxT, typename T>
 __global__ __launch_bounds__(1024) void face_area_sum_kernel(
-    const float* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
-    const float* __restrict__ coeff, double* __restrict__ cA, double* __restrict__ S_out,
+    const T* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
+    const T* __restrict__ coeff, double* __restrict__ cA, double* __restrict__ S_out,
     int* flag) {
   __shared__ double red[1024];
   __shared__ double redv[1024];
   double s = 0.0, vs = 0.0;
   for (int64_t t = threadIdx.x; t < nf; t += 1024) {
-    float P[3][3];
+    double P[3][3];
     load_face_verts(V, nv, Fc, t, P, flag);
     double u[3], q[3];
 #pragma unroll
@@ -134,23 +135,23 @@ __global__ __launch_bounds__(1024) void face_area_sum_kernel(
   }
 }
 
-template <typename IdxT>
+template <typename IdxT, typename T>
 __global__ __launch_bounds__(256) void face_pack_kernel(
-    const float* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
+    const T* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
     const double* __restrict__ cA, const double* __restrict__ S_in, float regint,
     FaceRec* __restrict__ out, int* flag) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= nf) return;
-  float P[3][3];
+  double P[3][3];
   load_face_verts(V, nv, Fc, t, P, flag);
   FaceRec r;
   const double S = S_in[0];
   const double regvol = S_in[1];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    r.a[d] = (double)P[0][d] * kInv2Pi;
-    r.eb[d] = P[1][d] - P[0][d];
-    r.ec[d] = P[2][d] - P[0][d];
+    r.a[d] = P[0][d] * kInv2Pi;
+    r.eb[d] = (float)(P[1][d] - P[0][d]);  // exact in FP64, rounded once
+    r.ec[d] = (float)(P[2][d] - P[0][d]);
   }
   r.w = (S > 0.0) ? (float)(-4.0 * cA[t] / S) : 0.0f;
   // autofluorescence: nor2_t = nor_t regint / regvol, nor_t = a x b + b x c + c x a
@@ -247,22 +248,22 @@ __device__ __forceinline__ void fwd_tile(const FaceRec* tile, int cnt, bool may_
 // lane also sums (xi . an_t) F_t and adds i/|xi|^2 times it at the end; at
 // xi = 0 that term is its limit -regint (the solid's transform at 0 over
 // regvol; the reference divides by zero there).
-template <bool kAF>
+template <bool kAF, typename T>
 __global__ __launch_bounds__(256) void meshft_fwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, int64_t faces_per_split,
-    const float* __restrict__ xi, int64_t n, float2* __restrict__ out,
+    const T* __restrict__ xi, int64_t n, T* __restrict__ out /* [n][2] */,
     double2* __restrict__ part, const double* __restrict__ S_in, float regint) {
   __shared__ FaceRec tile[kTile];
   __shared__ float red[4];
   const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool active = m < n;
-  float x = 0.0f, y = 0.0f, z = 0.0f;
+  double X = 0.0, Y = 0.0, Z = 0.0;  // the band frequency, exact (float or double input)
   if (active) {
-    x = xi[3 * m];
-    y = xi[3 * m + 1];
-    z = xi[3 * m + 2];
+    X = xi[3 * m];
+    Y = xi[3 * m + 1];
+    Z = xi[3 * m + 2];
   }
-  const double X = x, Y = y, Z = z;
+  const float x = (float)X, y = (float)Y, z = (float)Z;
   const float x2max = block_max256(x * x + y * y + z * z, red);
   const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
   const int64_t f1 = (f0 + faces_per_split < nf) ? f0 + faces_per_split : nf;
@@ -290,16 +291,19 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
     }
   }
   if (active) {
-    if (part)
+    if (part) {
       part[(int64_t)blockIdx.y * n + m] = make_double2(accr, acci);
-    else
-      out[m] = make_float2((float)accr, (float)acci);
+    } else {
+      out[2 * m] = (T)accr;
+      out[2 * m + 1] = (T)acci;
+    }
   }
 }
 
+template <typename T>
 __global__ __launch_bounds__(256) void fwd_split_reduce_kernel(const double2* __restrict__ part,
                                                                int nsplit, int64_t n,
-                                                               float2* __restrict__ out) {
+                                                               T* __restrict__ out) {
   const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (m >= n) return;
   double re = 0.0, im = 0.0;
@@ -308,7 +312,8 @@ __global__ __launch_bounds__(256) void fwd_split_reduce_kernel(const double2* __
     re += v.x;
     im += v.y;
   }
-  out[m] = make_float2((float)re, (float)im);
+  out[2 * m] = (T)re;
+  out[2 * m + 1] = (T)im;
 }
 
 // -------------------------------------------------------------------- adjoint
@@ -400,10 +405,10 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, bool may_
   }
 }
 
-template <bool kAF>
+template <bool kAF, typename T>
 __global__ __launch_bounds__(256) void meshft_bwd_kernel(
-    const FaceRec* __restrict__ faces, int64_t nf, const float* __restrict__ xi,
-    const float* __restrict__ grad, int64_t n, int64_t freqs_per_chunk,
+    const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi,
+    const T* __restrict__ grad, int64_t n, int64_t freqs_per_chunk,
     double* __restrict__ part /* [nchunk][kFaceSums][nf] */) {
   constexpr int K = kFaceSums<kAF>;
   __shared__ FreqRec tile[kTile];
@@ -432,10 +437,10 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
     if ((int)threadIdx.x < cnt) {
       const int64_t m = base + threadIdx.x;
       FreqRec fr;
-      fr.x[0] = xi[3 * m]; fr.x[1] = xi[3 * m + 1]; fr.x[2] = xi[3 * m + 2];
-      fr.X[0] = fr.x[0]; fr.X[1] = fr.x[1]; fr.X[2] = fr.x[2];
-      fr.gr = grad[2 * m];
-      fr.gi = grad[2 * m + 1];
+      fr.X[0] = xi[3 * m]; fr.X[1] = xi[3 * m + 1]; fr.X[2] = xi[3 * m + 2];
+      fr.x[0] = (float)fr.X[0]; fr.x[1] = (float)fr.X[1]; fr.x[2] = (float)fr.X[2];
+      fr.gr = (float)grad[2 * m];
+      fr.gi = (float)grad[2 * m + 1];
       const double q = fr.X[0] * fr.X[0] + fr.X[1] * fr.X[1] + fr.X[2] * fr.X[2];
       fr.inv2 = (kAF && q > 0.0) ? (float)(1.0 / q) : 0.0f;
       tile[threadIdx.x] = fr;
@@ -462,10 +467,10 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
 //   cgV_tj = d regvol / dv_j = -(v_{j+1} x v_{j+2}) / 6,  qN_t = nor_t . N_t
 // (nor_t . N_t is the derivative w.r.t. nor_t; its regvol dependence is the
 // global term -(regint / regvol^2) sum_t qN_t cgV_tj, added in the gather).
-template <typename IdxT, bool kAF>
+template <typename IdxT, bool kAF, typename T>
 __global__ __launch_bounds__(256) void bwd_face_finalize_kernel(
-    const float* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
-    const float* __restrict__ coeff, const double* __restrict__ cA, const double* __restrict__ S_in,
+    const T* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
+    const T* __restrict__ coeff, const double* __restrict__ cA, const double* __restrict__ S_in,
     const double* __restrict__ part, int nchunk, float regint, double* __restrict__ G,
     double* __restrict__ cgA, double* __restrict__ wR, double* __restrict__ cgV,
     double* __restrict__ qN, int* flag) {
@@ -480,13 +485,8 @@ __global__ __launch_bounds__(256) void bwd_face_finalize_kernel(
 #pragma unroll
     for (int i = 0; i < K; ++i) s[i] += src[(int64_t)i * nf];
   }
-  float Pf[3][3];
-  load_face_verts(V, nv, Fc, t, Pf, flag);
   double P[3][3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) P[j][d] = (double)Pf[j][d];
+  load_face_verts(V, nv, Fc, t, P, flag);
   double u[3], q[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -635,11 +635,12 @@ __global__ __launch_bounds__(256) void csr_sort_rows_kernel(const int* __restric
 // grad_v = sum over incidences (t, j) of G_tj - (K/S) cgA_tj, and for the
 // autofluorescence variant (cgV != null) - (regint Kv / regvol^2) cgV_tj
 // with Kv = sum_t qN_t.
+template <typename T>
 __global__ __launch_bounds__(256) void vertex_gather_kernel(
     const int* __restrict__ row, const int* __restrict__ ent, int64_t nv,
     const double* __restrict__ G, const double* __restrict__ cgA, const double* __restrict__ K_in,
     const double* __restrict__ S_in, const double* __restrict__ cgV,
-    const double* __restrict__ Kv_in, float regint, float* __restrict__ grad) {
+    const double* __restrict__ Kv_in, float regint, T* __restrict__ grad) {
   const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (v >= nv) return;
   const double S = S_in[0];
@@ -658,9 +659,9 @@ __global__ __launch_bounds__(256) void vertex_gather_kernel(
       g2 -= kv * cgV[3 * e + 2];
     }
   }
-  grad[3 * v] = (float)g0;
-  grad[3 * v + 1] = (float)g1;
-  grad[3 * v + 2] = (float)g2;
+  grad[3 * v] = (T)g0;
+  grad[3 * v + 1] = (T)g1;
+  grad[3 * v + 2] = (T)g2;
 }
 
 // -------------------------------------------------------------------- host side
@@ -806,22 +807,23 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n, bool af) {
   return w;
 }
 
-template <typename IdxT>
-static int prep_faces(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, const float* coeff,
+template <typename IdxT, typename T>
+static int prep_faces(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const T* coeff,
                       float regint, int* flag, double* cA, double* S, FaceRec* recs,
                       hipStream_t st) {
-  hipLaunchKernelGGL(face_area_sum_kernel<IdxT>, dim3(1), dim3(1024), 0, st, V, nv, Fc, nf, coeff,
-                     cA, S, flag);
+  hipLaunchKernelGGL((face_area_sum_kernel<IdxT, T>), dim3(1), dim3(1024), 0, st, V, nv, Fc, nf,
+                     coeff, cA, S, flag);
   DMIC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(face_pack_kernel<IdxT>, dim3(ceil_div(nf, 256)), dim3(256), 0, st, V, nv, Fc,
-                     nf, cA, S, regint, recs, flag);
+  hipLaunchKernelGGL((face_pack_kernel<IdxT, T>), dim3(ceil_div(nf, 256)), dim3(256), 0, st, V, nv,
+                     Fc, nf, cA, S, regint, recs, flag);
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
 }
 
-template <typename IdxT, bool kAF>
-static int forward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, const float* coeff,
-                        float regint, const float* xi, int64_t n, float* out, void* ws,
+// T: the boundary's floating type (float, or double for the _f64 entry points).
+template <typename IdxT, bool kAF, typename T>
+static int forward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const T* coeff,
+                        float regint, const T* xi, int64_t n, T* out, void* ws,
                         size_t ws_bytes, hipStream_t st, int64_t part_cap = -1) {
   Carver c(ws);
   FwdWs w = carve_fwd(c, nf, n, part_cap);
@@ -830,37 +832,37 @@ static int forward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf, 
   hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, w.flag, 4, 0);
   DMIC_LAUNCH_CHECK();
   if (n == 0) return DMIC_OK;
+  constexpr int64_t kWords = sizeof(T) / sizeof(int);  // 32-bit words per value
   if (nf == 0) {
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * n, 256)), dim3(256), 0, st,
-                       reinterpret_cast<int*>(out), 2 * n, 0);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * kWords * n, 256)), dim3(256), 0, st,
+                       reinterpret_cast<int*>(out), 2 * kWords * n, 0);
     DMIC_LAUNCH_CHECK();
     return DMIC_OK;
   }
-  int rc = prep_faces<IdxT>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S, w.recs,
-                            st);
+  int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S,
+                               w.recs, st);
   if (rc != DMIC_OK) return rc;
   const FwdPlan p = fwd_plan(nf, n, part_cap);
   dim3 grid(ceil_div(n, 256), p.nsplit);
   rc = timer_begin(0, st);
   if (rc != DMIC_OK) return rc;
-  hipLaunchKernelGGL(meshft_fwd_kernel<kAF>, grid, dim3(256), 0, st, w.recs, nf, p.faces_per_split,
-                     xi, n, reinterpret_cast<float2*>(out), p.nsplit > 1 ? w.part : nullptr, w.S,
-                     regint);
+  hipLaunchKernelGGL((meshft_fwd_kernel<kAF, T>), grid, dim3(256), 0, st, w.recs, nf,
+                     p.faces_per_split, xi, n, out, p.nsplit > 1 ? w.part : nullptr, w.S, regint);
   DMIC_LAUNCH_CHECK();
   rc = timer_end(0, st);
   if (rc != DMIC_OK) return rc;
   if (p.nsplit > 1) {
-    hipLaunchKernelGGL(fwd_split_reduce_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, w.part,
-                       p.nsplit, n, reinterpret_cast<float2*>(out));
+    hipLaunchKernelGGL(fwd_split_reduce_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+                       w.part, p.nsplit, n, out);
     DMIC_LAUNCH_CHECK();
   }
   return DMIC_OK;
 }
 
-template <typename IdxT, bool kAF>
-static int backward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf,
-                         const float* coeff, float regint, const float* grad, const float* xi,
-                         int64_t n, float* gverts, void* ws, size_t ws_bytes, hipStream_t st) {
+template <typename IdxT, bool kAF, typename T>
+static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const T* coeff,
+                         float regint, const T* grad, const T* xi, int64_t n, T* gverts, void* ws,
+                         size_t ws_bytes, hipStream_t st) {
   constexpr int K = kFaceSums<kAF>;
   Carver c(ws);
   BwdWs w = carve_bwd(c, nv, nf, n, kAF);
@@ -869,22 +871,23 @@ static int backward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf,
   hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, w.flag, 4, 0);
   DMIC_LAUNCH_CHECK();
   if (nv == 0) return DMIC_OK;
+  constexpr int64_t kWords = sizeof(T) / sizeof(int);  // 32-bit words per value
   if (nf == 0) {
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(3 * nv, 256)), dim3(256), 0, st,
-                       reinterpret_cast<int*>(gverts), 3 * nv, 0);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(3 * kWords * nv, 256)), dim3(256), 0, st,
+                       reinterpret_cast<int*>(gverts), 3 * kWords * nv, 0);
     DMIC_LAUNCH_CHECK();
     return DMIC_OK;
   }
-  int rc = prep_faces<IdxT>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S, w.recs,
-                            st);
+  int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S,
+                               w.recs, st);
   if (rc != DMIC_OK) return rc;
   const BwdPlan p = bwd_plan(nf, n);
   if (n > 0) {
     dim3 grid(ceil_div(nf, 256), p.nchunk);
     rc = timer_begin(1, st);
     if (rc != DMIC_OK) return rc;
-    hipLaunchKernelGGL(meshft_bwd_kernel<kAF>, grid, dim3(256), 0, st, w.recs, nf, xi, grad, n,
-                       p.freqs_per_chunk, w.part);
+    hipLaunchKernelGGL((meshft_bwd_kernel<kAF, T>), grid, dim3(256), 0, st, w.recs, nf, xi, grad,
+                       n, p.freqs_per_chunk, w.part);
     DMIC_LAUNCH_CHECK();
     rc = timer_end(1, st);
     if (rc != DMIC_OK) return rc;
@@ -894,7 +897,7 @@ static int backward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf,
     DMIC_LAUNCH_CHECK();
   }
   const int nchunk = (n > 0) ? p.nchunk : 1;
-  hipLaunchKernelGGL((bwd_face_finalize_kernel<IdxT, kAF>), dim3(ceil_div(nf, 256)), dim3(256), 0,
+  hipLaunchKernelGGL((bwd_face_finalize_kernel<IdxT, kAF, T>), dim3(ceil_div(nf, 256)), dim3(256), 0,
                      st, V, nv, Fc, nf, coeff, w.cA, w.S, w.part, nchunk, regint, w.G, w.cgA, w.wR,
                      w.cgV, w.qN, w.flag);
   DMIC_LAUNCH_CHECK();
@@ -920,8 +923,8 @@ static int backward_impl(const float* V, int64_t nv, const IdxT* Fc, int64_t nf,
   hipLaunchKernelGGL(csr_sort_rows_kernel, dim3(ceil_div(nv, 256)), dim3(256), 0, st, w.row, nv,
                      w.ent);
   DMIC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(vertex_gather_kernel, dim3(ceil_div(nv, 256)), dim3(256), 0, st, w.row, w.ent,
-                     nv, w.G, w.cgA, w.K, w.S, w.cgV, w.Kv, regint, gverts);
+  hipLaunchKernelGGL(vertex_gather_kernel<T>, dim3(ceil_div(nv, 256)), dim3(256), 0, st, w.row,
+                     w.ent, nv, w.G, w.cgA, w.K, w.S, w.cgV, w.Kv, regint, gverts);
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
 }
@@ -962,10 +965,10 @@ size_t dmic_meshft_af_backward_workspace(int64_t nv, int64_t nf, int64_t n) {
 
 namespace {
 
-template <bool kAF>
-int forward_entry(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
-                  int64_t nf, const float* faces_coeff, float regint, const float* xi_list,
-                  int64_t n, float* ftmesh_list, void* workspace, size_t workspace_bytes,
+template <bool kAF, typename T>
+int forward_entry(const T* verts, int64_t nv, const void* faces, int faces_is_int64,
+                  int64_t nf, const T* faces_coeff, float regint, const T* xi_list,
+                  int64_t n, T* ftmesh_list, void* workspace, size_t workspace_bytes,
                   hipStream_t stream) {
   g_err[0] = 0;
   DMIC_REQUIRE(nv >= 0 && nf >= 0 && n >= 0, DMIC_ERR_ARG, "negative size");
@@ -975,20 +978,19 @@ int forward_entry(const float* verts, int64_t nv, const void* faces, int faces_i
                "null mesh pointer");
   DMIC_REQUIRE(workspace || workspace_bytes == 0, DMIC_ERR_ARG, "null workspace");
   if (faces_is_int64)
-    return forward_impl<int64_t, kAF>(verts, nv, static_cast<const int64_t*>(faces), nf,
-                                      faces_coeff, regint, xi_list, n, ftmesh_list, workspace,
-                                      workspace_bytes, stream);
-  return forward_impl<int32_t, kAF>(verts, nv, static_cast<const int32_t*>(faces), nf, faces_coeff,
-                                    regint, xi_list, n, ftmesh_list, workspace, workspace_bytes,
-                                    stream);
+    return forward_impl<int64_t, kAF, T>(verts, nv, static_cast<const int64_t*>(faces), nf,
+                                         faces_coeff, regint, xi_list, n, ftmesh_list, workspace,
+                                         workspace_bytes, stream);
+  return forward_impl<int32_t, kAF, T>(verts, nv, static_cast<const int32_t*>(faces), nf,
+                                       faces_coeff, regint, xi_list, n, ftmesh_list, workspace,
+                                       workspace_bytes, stream);
 }
 
-template <bool kAF>
-int backward_entry(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
-                   int64_t nf, const float* faces_coeff, float regint,
-                   const float* grad_output_list, const float* xi_list, int64_t n,
-                   float* grad_verts, void* workspace, size_t workspace_bytes,
-                   hipStream_t stream) {
+template <bool kAF, typename T>
+int backward_entry(const T* verts, int64_t nv, const void* faces, int faces_is_int64,
+                   int64_t nf, const T* faces_coeff, float regint, const T* grad_output_list,
+                   const T* xi_list, int64_t n, T* grad_verts, void* workspace,
+                   size_t workspace_bytes, hipStream_t stream) {
   g_err[0] = 0;
   DMIC_REQUIRE(nv >= 0 && nf >= 0 && n >= 0, DMIC_ERR_ARG, "negative size");
   DMIC_REQUIRE(nf < (int64_t(1) << 31) / 3 && nv < (int64_t(1) << 31) && n < (int64_t(1) << 40),
@@ -999,12 +1001,12 @@ int backward_entry(const float* verts, int64_t nv, const void* faces, int faces_
                "null xi_list/grad_output_list");
   DMIC_REQUIRE(workspace || workspace_bytes == 0, DMIC_ERR_ARG, "null workspace");
   if (faces_is_int64)
-    return backward_impl<int64_t, kAF>(verts, nv, static_cast<const int64_t*>(faces), nf,
-                                       faces_coeff, regint, grad_output_list, xi_list, n,
-                                       grad_verts, workspace, workspace_bytes, stream);
-  return backward_impl<int32_t, kAF>(verts, nv, static_cast<const int32_t*>(faces), nf,
-                                     faces_coeff, regint, grad_output_list, xi_list, n,
-                                     grad_verts, workspace, workspace_bytes, stream);
+    return backward_impl<int64_t, kAF, T>(verts, nv, static_cast<const int64_t*>(faces), nf,
+                                          faces_coeff, regint, grad_output_list, xi_list, n,
+                                          grad_verts, workspace, workspace_bytes, stream);
+  return backward_impl<int32_t, kAF, T>(verts, nv, static_cast<const int32_t*>(faces), nf,
+                                        faces_coeff, regint, grad_output_list, xi_list, n,
+                                        grad_verts, workspace, workspace_bytes, stream);
 }
 
 }  // namespace
@@ -1069,20 +1071,28 @@ int dmic_meshft_forward_noalloc(const float* verts, int64_t nv, const void* face
   const int rc = scratch_for_current_device(&ws);
   if (rc != DMIC_OK) return rc;
   if (faces_is_int64)
-    return forward_impl<int64_t, false>(verts, nv, static_cast<const int64_t*>(faces), nf,
-                                        faces_coeff, 0.0f, xi_list, n, ftmesh_list, ws,
-                                        kNoAllocScratchBytes, stream, part_cap);
-  return forward_impl<int32_t, false>(verts, nv, static_cast<const int32_t*>(faces), nf,
-                                      faces_coeff, 0.0f, xi_list, n, ftmesh_list, ws,
-                                      kNoAllocScratchBytes, stream, part_cap);
+    return forward_impl<int64_t, false, float>(verts, nv, static_cast<const int64_t*>(faces), nf,
+                                               faces_coeff, 0.0f, xi_list, n, ftmesh_list, ws,
+                                               kNoAllocScratchBytes, stream, part_cap);
+  return forward_impl<int32_t, false, float>(verts, nv, static_cast<const int32_t*>(faces), nf,
+                                             faces_coeff, 0.0f, xi_list, n, ftmesh_list, ws,
+                                             kNoAllocScratchBytes, stream, part_cap);
 }
 
 int dmic_meshft_forward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
                         int64_t nf, const float* faces_coeff, const float* xi_list, int64_t n,
                         float* ftmesh_list, void* workspace, size_t workspace_bytes,
                         hipStream_t stream) {
-  return forward_entry<false>(verts, nv, faces, faces_is_int64, nf, faces_coeff, 0.0f, xi_list, n,
-                              ftmesh_list, workspace, workspace_bytes, stream);
+  return forward_entry<false, float>(verts, nv, faces, faces_is_int64, nf, faces_coeff, 0.0f,
+                                     xi_list, n, ftmesh_list, workspace, workspace_bytes, stream);
+}
+
+int dmic_meshft_forward_f64(const double* verts, int64_t nv, const void* faces,
+                            int faces_is_int64, int64_t nf, const double* faces_coeff,
+                            const double* xi_list, int64_t n, double* ftmesh_list,
+                            void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  return forward_entry<false, double>(verts, nv, faces, faces_is_int64, nf, faces_coeff, 0.0f,
+                                      xi_list, n, ftmesh_list, workspace, workspace_bytes, stream);
 }
 
 int dmic_meshft_backward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
@@ -1091,17 +1101,28 @@ int dmic_meshft_backward(const float* verts, int64_t nv, const void* faces, int 
                          float* grad_verts, void* workspace, size_t workspace_bytes,
                          hipStream_t stream) {
   (void)ftmesh_list;
-  return backward_entry<false>(verts, nv, faces, faces_is_int64, nf, faces_coeff, 0.0f,
-                               grad_output_list, xi_list, n, grad_verts, workspace,
-                               workspace_bytes, stream);
+  return backward_entry<false, float>(verts, nv, faces, faces_is_int64, nf, faces_coeff, 0.0f,
+                                      grad_output_list, xi_list, n, grad_verts, workspace,
+                                      workspace_bytes, stream);
+}
+
+int dmic_meshft_backward_f64(const double* verts, int64_t nv, const void* faces,
+                             int faces_is_int64, int64_t nf, const double* faces_coeff,
+                             const double* grad_output_list, const double* ftmesh_list,
+                             const double* xi_list, int64_t n, double* grad_verts,
+                             void* workspace, size_t workspace_bytes, hipStream_t stream) {
+  (void)ftmesh_list;
+  return backward_entry<false, double>(verts, nv, faces, faces_is_int64, nf, faces_coeff, 0.0f,
+                                       grad_output_list, xi_list, n, grad_verts, workspace,
+                                       workspace_bytes, stream);
 }
 
 int dmic_meshft_af_forward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
                            int64_t nf, const float* faces_coeff, float regint,
                            const float* xi_list, int64_t n, float* ftmesh_list, void* workspace,
                            size_t workspace_bytes, hipStream_t stream) {
-  return forward_entry<true>(verts, nv, faces, faces_is_int64, nf, faces_coeff, regint, xi_list, n,
-                             ftmesh_list, workspace, workspace_bytes, stream);
+  return forward_entry<true, float>(verts, nv, faces, faces_is_int64, nf, faces_coeff, regint,
+                                    xi_list, n, ftmesh_list, workspace, workspace_bytes, stream);
 }
 
 int dmic_meshft_af_backward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
@@ -1109,9 +1130,9 @@ int dmic_meshft_af_backward(const float* verts, int64_t nv, const void* faces, i
                             const float* grad_output_list, const float* xi_list, int64_t n,
                             float* grad_verts, void* workspace, size_t workspace_bytes,
                             hipStream_t stream) {
-  return backward_entry<true>(verts, nv, faces, faces_is_int64, nf, faces_coeff, regint,
-                              grad_output_list, xi_list, n, grad_verts, workspace,
-                              workspace_bytes, stream);
+  return backward_entry<true, float>(verts, nv, faces, faces_is_int64, nf, faces_coeff, regint,
+                                     grad_output_list, xi_list, n, grad_verts, workspace,
+                                     workspace_bytes, stream);
 }
 
 int dmic_profile_enable(int enable) {

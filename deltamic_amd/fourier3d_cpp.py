@@ -11,7 +11,10 @@ Differences from the reference, all on purpose:
   * an empty band (n = 0) or an empty mesh returns zeros instead of a
     zero-block launch;
   * any vertex valence works (the reference caps it at 30, .cu:487);
-  * only float32 is implemented (the reference also dispatches float64).
+  * float64 inputs (the reference's AT_DISPATCH_FLOATING_TYPES double
+    case) go to the _f64 entry points: FP64 vertices and frequencies at the
+    boundary, complex128 / float64 results, the same FP32 arithmetic with
+    FP64 phases inside as for float32 (accuracy of the float32 path).
 """
 from __future__ import annotations
 
@@ -32,10 +35,11 @@ def _check_mesh(mesh_vertices, mesh_faces, faces_coeff):
     _check_input(mesh_vertices, "mesh_vertices")
     _check_input(mesh_faces, "mesh_faces")
     _check_input(faces_coeff, "faces_coeff")
-    if mesh_vertices.dtype != torch.float32:
-        raise TypeError(f"meshFT is implemented for float32 vertices, got {mesh_vertices.dtype}")
-    if faces_coeff.dtype != torch.float32:
-        raise TypeError(f"faces_coeff must be float32, got {faces_coeff.dtype}")
+    if mesh_vertices.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"meshFT takes float32 or float64 vertices, got {mesh_vertices.dtype}")
+    if faces_coeff.dtype != mesh_vertices.dtype:
+        raise TypeError(f"faces_coeff must be {mesh_vertices.dtype} like the vertices, got "
+                        f"{faces_coeff.dtype}")
     if mesh_faces.dtype not in (torch.int64, torch.int32):
         raise TypeError(f"mesh_faces must be int64 or int32, got {mesh_faces.dtype}")
     if mesh_vertices.dim() != 2 or mesh_vertices.shape[1] != 3:
@@ -79,21 +83,29 @@ def forward_fourier3d_cpp(mesh_vertices: torch.Tensor, mesh_faces: torch.Tensor,
     w_t + i (xi . nor2_t) / |xi|^2 with nor2 = nor regint / regvol."""
     _check_mesh(mesh_vertices, mesh_faces, faces_coeff)
     _check_input(xi_list, "xi_list")
-    if xi_list.dtype != torch.float32 or xi_list.dim() != 2 or xi_list.shape[1] != 3:
-        raise TypeError("xi_list must be float32 [n, 3]")
+    dt = mesh_vertices.dtype
+    if xi_list.dtype != dt or xi_list.dim() != 2 or xi_list.shape[1] != 3:
+        raise TypeError(f"xi_list must be {dt} [n, 3] like the vertices")
+    ct = torch.complex64 if dt == torch.float32 else torch.complex128
     nv, nf, n = mesh_vertices.shape[0], mesh_faces.shape[0], xi_list.shape[0]
     validate_faces(mesh_faces, nv)
     dev = mesh_vertices.device
     if out is None:
-        out = torch.empty((n,), dtype=torch.complex64, device=dev)
-    elif (out.dtype != torch.complex64 or not out.is_contiguous() or out.numel() < n
-          or out.device != dev):
-        raise ValueError("out must be a contiguous complex64 tensor with at least n entries")
+        out = torch.empty((n,), dtype=ct, device=dev)
+    elif (out.dtype != ct or not out.is_contiguous() or out.numel() < n or out.device != dev):
+        raise ValueError(f"out must be a contiguous {ct} tensor with at least n entries")
     L = _lib.lib()
     nbytes = L.dmic_meshft_forward_workspace(nv, nf, n)
     ws = _lib.workspace(nbytes, dev)
     is64 = int(mesh_faces.dtype == torch.int64)
-    if regint is None:
+    if dt == torch.float64:
+        if regint is not None:
+            raise TypeError("the autofluorescence variant takes float32 inputs")
+        rc = L.dmic_meshft_forward_f64(
+            mesh_vertices.data_ptr(), nv, mesh_faces.data_ptr(), is64, nf, faces_coeff.data_ptr(),
+            xi_list.data_ptr(), n, out.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_of(out))
+        _lib.check(rc, "dmic_meshft_forward_f64")
+    elif regint is None:
         rc = L.dmic_meshft_forward(
             mesh_vertices.data_ptr(), nv, mesh_faces.data_ptr(), is64, nf, faces_coeff.data_ptr(),
             xi_list.data_ptr(), n, out.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_of(out))
@@ -117,18 +129,30 @@ def backward_fourier3d_cpp(mesh_vertices: torch.Tensor, mesh_faces: torch.Tensor
     _check_input(grad_output_list, "grad_output_list")
     _check_input(ftmesh_list, "ftmesh_list")
     _check_input(xi_list, "xi_list")
-    if xi_list.dtype != torch.float32 or xi_list.dim() != 2 or xi_list.shape[1] != 3:
-        raise TypeError("xi_list must be float32 [n, 3]")
+    dt = mesh_vertices.dtype
+    if xi_list.dtype != dt or xi_list.dim() != 2 or xi_list.shape[1] != 3:
+        raise TypeError(f"xi_list must be {dt} [n, 3] like the vertices")
+    ct = torch.complex64 if dt == torch.float32 else torch.complex128
     n = xi_list.shape[0]
-    if grad_output_list.dtype != torch.complex64 or grad_output_list.shape != (n,):
-        raise TypeError("grad_output_list must be complex64 [n]")
+    if grad_output_list.dtype != ct or grad_output_list.shape != (n,):
+        raise TypeError(f"grad_output_list must be {ct} [n]")
     nv, nf = mesh_vertices.shape[0], mesh_faces.shape[0]
     validate_faces(mesh_faces, nv)
     dev = mesh_vertices.device
-    grad = torch.empty((nv, 3), dtype=torch.float32, device=dev)
+    grad = torch.empty((nv, 3), dtype=dt, device=dev)
     L = _lib.lib()
     is64 = int(mesh_faces.dtype == torch.int64)
-    if regint is None:
+    if dt == torch.float64:
+        if regint is not None:
+            raise TypeError("the autofluorescence variant takes float32 inputs")
+        nbytes = L.dmic_meshft_backward_workspace(nv, nf, n)
+        ws = _lib.workspace(nbytes, dev)
+        rc = L.dmic_meshft_backward_f64(
+            mesh_vertices.data_ptr(), nv, mesh_faces.data_ptr(), is64, nf, faces_coeff.data_ptr(),
+            grad_output_list.data_ptr(), ftmesh_list.data_ptr(), xi_list.data_ptr(), n,
+            grad.data_ptr(), ws.data_ptr(), ws.numel(), _lib.stream_of(grad))
+        _lib.check(rc, "dmic_meshft_backward_f64")
+    elif regint is None:
         nbytes = L.dmic_meshft_backward_workspace(nv, nf, n)
         ws = _lib.workspace(nbytes, dev)
         rc = L.dmic_meshft_backward(

@@ -53,6 +53,21 @@ class Band:
     symmetric: bool = False  # entry n-1-j is exactly -xi of entry j
     closure: "Band | None" = None   # symmetric closure B u -B of a non-centrosymmetric band
     pos: torch.Tensor | None = None  # int32 [n]: position of each entry in the closure
+    axes: tuple = ()                 # the frequency axes the band was built from
+    _xi64: torch.Tensor | None = None
+
+    def freqs(self, dtype) -> torch.Tensor:
+        """Band frequencies [n, 3] in ``dtype``: float32 as compacted, float64
+        gathered from the axes in FP64 (exact for float32 axes)."""
+        if dtype == torch.float32:
+            return self.xi_list
+        if self._xi64 is None:
+            i = self.idx.long()
+            n1, n2 = self.shape[1], self.shape[2]
+            a0, a1, a2 = (a.to(device=self.idx.device, dtype=torch.float64) for a in self.axes)
+            self._xi64 = torch.stack([a0[i // (n1 * n2)], a1[(i // n2) % n1], a2[i % n2]],
+                                     dim=1).contiguous()
+        return self._xi64
 
     @property
     def n(self) -> int:
@@ -61,6 +76,10 @@ class Band:
     @property
     def half(self) -> int:
         return (self.n + 1) // 2
+
+    @property
+    def size(self) -> int:
+        return self.shape[0] * self.shape[1] * self.shape[2]
 
 
 def _thresh_value(narrowband_thresh) -> float:
@@ -118,8 +137,9 @@ def compute_band(Filter: torch.Tensor, thresh: float, xi0: torch.Tensor, xi1: to
             if cb.symmetric:
                 pos = torch.searchsorted(cb.idx, idx, out_int32=True)
                 return Band(idx=idx, xi_list=xi_list, shape=(n0, n1, n2), key=(),
-                            symmetric=False, closure=cb, pos=pos)
-    return Band(idx=idx, xi_list=xi_list, shape=(n0, n1, n2), key=(), symmetric=symmetric)
+                            symmetric=False, closure=cb, pos=pos, axes=(xi0, xi1, xi2))
+    return Band(idx=idx, xi_list=xi_list, shape=(n0, n1, n2), key=(), symmetric=symmetric,
+                axes=(xi0, xi1, xi2))
 
 
 def band_for(Filter: torch.Tensor, narrowband_thresh, xi0, xi1, xi2, cache: bool = True) -> Band:
@@ -146,24 +166,66 @@ def clear_band_cache(Filter: torch.Tensor) -> None:
         del Filter._dmic_band
 
 
+def _scatter(vals: torch.Tensor, idx: torch.Tensor, size: int) -> torch.Tensor:
+    """[size] zeros with out[idx[j]] = vals[j] (complex64 by the HIP kernel)."""
+    if vals.dtype == torch.complex64:
+        out = torch.empty((size,), dtype=torch.complex64, device=vals.device)
+        L = _lib.lib()
+        _lib.check(L.dmic_band_scatter(vals.data_ptr(), idx.data_ptr(), idx.numel(),
+                                       out.data_ptr(), size, _lib.stream_of(out)),
+                   "dmic_band_scatter")
+        return out
+    out = torch.zeros((size,), dtype=vals.dtype, device=vals.device)
+    out[idx.long()] = vals
+    return out
+
+
+def _gather(src: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+    """src[idx] (complex64 by the HIP kernel)."""
+    src = src.contiguous()
+    if src.dtype == torch.complex64:
+        vals = torch.empty((idx.numel(),), dtype=torch.complex64, device=src.device)
+        L = _lib.lib()
+        _lib.check(L.dmic_band_gather(src.data_ptr(), idx.data_ptr(), idx.numel(),
+                                      vals.data_ptr(), _lib.stream_of(vals)), "dmic_band_gather")
+        return vals
+    return src.reshape(-1)[idx.long()]
+
+
 def scatter_band(vals: torch.Tensor, band: Band) -> torch.Tensor:
-    box = torch.empty(band.shape, dtype=torch.complex64, device=vals.device)
-    L = _lib.lib()
-    _lib.check(L.dmic_band_scatter(vals.data_ptr(), band.idx.data_ptr(), band.idx.numel(),
-                                   box.data_ptr(), box.numel(), _lib.stream_of(box)),
-               "dmic_band_scatter")
-    return box
+    return _scatter(vals, band.idx, band.size).reshape(band.shape)
 
 
-def gather_band(box: torch.Tensor, band: Band) -> torch.Tensor:
-    box = box.contiguous()
-    if box.dtype != torch.complex64:
-        box = box.to(torch.complex64)
-    vals = torch.empty((band.idx.numel(),), dtype=torch.complex64, device=box.device)
-    L = _lib.lib()
-    _lib.check(L.dmic_band_gather(box.data_ptr(), band.idx.data_ptr(), band.idx.numel(),
-                                  vals.data_ptr(), _lib.stream_of(box)), "dmic_band_gather")
-    return vals
+def gather_band(box: torch.Tensor, band: Band, dtype=torch.complex64) -> torch.Tensor:
+    if box.dtype != dtype:
+        box = box.to(dtype)
+    return _gather(box, band.idx)
+
+
+def _mirror_fill(out: torch.Tensor, n: int) -> None:
+    """out[n-1-j] = conj(out[j]) for j < n/2."""
+    if out.dtype == torch.complex64:
+        L = _lib.lib()
+        _lib.check(L.dmic_band_mirror_fill(out.data_ptr(), n, _lib.stream_of(out)),
+                   "dmic_band_mirror_fill")
+        return
+    h = n // 2
+    out[n - h:n] = torch.conj(out[:h]).flip(0)
+
+
+def _mirror_fold(g: torch.Tensor, n: int) -> torch.Tensor:
+    """g_half[j] = g[j] + conj(g[n-1-j]) for j < n/2 (the middle entry of an odd band copied)."""
+    half = (n + 1) // 2
+    if g.dtype == torch.complex64:
+        g_half = torch.empty((half,), dtype=torch.complex64, device=g.device)
+        L = _lib.lib()
+        _lib.check(L.dmic_band_mirror_fold(g.data_ptr(), n, g_half.data_ptr(),
+                                           _lib.stream_of(g_half)), "dmic_band_mirror_fold")
+        return g_half
+    h = n // 2
+    g_half = g[:half].clone()
+    g_half[:h] += torch.conj(g[n - h:n]).flip(0)
+    return g_half
 
 
 def band_forward(V, Fc, C, band: Band, regint=None) -> torch.Tensor:
@@ -171,19 +233,14 @@ def band_forward(V, Fc, C, band: Band, regint=None) -> torch.Tensor:
     gathered from the half-evaluated closure when the band has one; both
     variants are Hermitian: real face weights and xi . nor2 odd in xi)."""
     if HERMITIAN_HALVING and band.closure is not None:
-        vals_c = band_forward(V, Fc, C, band.closure, regint=regint)
-        out = torch.empty((band.n,), dtype=torch.complex64, device=V.device)
-        L = _lib.lib()
-        _lib.check(L.dmic_band_gather(vals_c.data_ptr(), band.pos.data_ptr(), band.n,
-                                      out.data_ptr(), _lib.stream_of(out)), "dmic_band_gather")
-        return out
+        return _gather(band_forward(V, Fc, C, band.closure, regint=regint), band.pos)
+    xi = band.freqs(V.dtype)
     if not (HERMITIAN_HALVING and band.symmetric and band.n > 1):
-        return forward_fourier3d_cpp(V, Fc, C, band.xi_list, regint=regint)
-    out = torch.empty((band.n,), dtype=torch.complex64, device=V.device)
-    forward_fourier3d_cpp(V, Fc, C, band.xi_list[:band.half], out=out, regint=regint)
-    L = _lib.lib()
-    _lib.check(L.dmic_band_mirror_fill(out.data_ptr(), band.n, _lib.stream_of(out)),
-               "dmic_band_mirror_fill")
+        return forward_fourier3d_cpp(V, Fc, C, xi, regint=regint)
+    ct = torch.complex64 if V.dtype == torch.float32 else torch.complex128
+    out = torch.empty((band.n,), dtype=ct, device=V.device)
+    forward_fourier3d_cpp(V, Fc, C, xi[:band.half], out=out, regint=regint)
+    _mirror_fill(out, band.n)
     return out
 
 
@@ -192,23 +249,16 @@ def band_backward(V, Fc, C, grad_list, ftmesh_list, band: Band, regint=None) -> 
     scattered into the closure first when the band has one)."""
     if HERMITIAN_HALVING and band.closure is not None:
         cb = band.closure
-        g_c = torch.empty((cb.n,), dtype=torch.complex64, device=V.device)
-        L = _lib.lib()
-        _lib.check(L.dmic_band_scatter(grad_list.data_ptr(), band.pos.data_ptr(), band.n,
-                                       g_c.data_ptr(), cb.n, _lib.stream_of(g_c)),
-                   "dmic_band_scatter")
+        g_c = _scatter(grad_list, band.pos, cb.n)
         # the adjoint does not read ftmesh_list (it rebuilds the area term from
         # its own per-face sums), so the closure gradient stands in for it
         return band_backward(V, Fc, C, g_c, g_c, cb, regint=regint)
+    xi = band.freqs(V.dtype)
     if not (HERMITIAN_HALVING and band.symmetric and band.n > 1):
-        return backward_fourier3d_cpp(V, Fc, C, grad_list, ftmesh_list, band.xi_list,
-                                      regint=regint)
-    g_half = torch.empty((band.half,), dtype=torch.complex64, device=V.device)
-    L = _lib.lib()
-    _lib.check(L.dmic_band_mirror_fold(grad_list.data_ptr(), band.n, g_half.data_ptr(),
-                                       _lib.stream_of(g_half)), "dmic_band_mirror_fold")
-    return backward_fourier3d_cpp(V, Fc, C, g_half, ftmesh_list[:band.half],
-                                  band.xi_list[:band.half], regint=regint)
+        return backward_fourier3d_cpp(V, Fc, C, grad_list, ftmesh_list, xi, regint=regint)
+    g_half = _mirror_fold(grad_list, band.n)
+    return backward_fourier3d_cpp(V, Fc, C, g_half, ftmesh_list[:band.half], xi[:band.half],
+                                  regint=regint)
 
 
 class MeshFTHIP(Function):
@@ -217,25 +267,26 @@ class MeshFTHIP(Function):
     @staticmethod
     def forward(ctx, Verts, Faces, Faces_coeff, xi0, xi1, xi2, Filter, narrowband_thresh,
                 cache_band=True):
-        if Verts.dtype != torch.float32:
-            raise TypeError("Mesh FT (HIP) is implemented for float32 only, got " + str(Verts.dtype))
+        if Verts.dtype not in (torch.float32, torch.float64):
+            raise TypeError("Mesh FT (HIP) takes float32 or float64 vertices, got "
+                            + str(Verts.dtype))
         band = band_for(Filter, narrowband_thresh, xi0, xi1, xi2, cache=cache_band)
         V = Verts.detach().contiguous()
         Fc = Faces.contiguous()
-        C = Faces_coeff.detach().to(torch.float32).contiguous()
+        C = Faces_coeff.detach().to(V.dtype).contiguous()
         ftmesh_list = band_forward(V, Fc, C, band)
         ctx.save_for_backward(V, Fc, C, ftmesh_list)
         ctx.band = band
         ftmesh = scatter_band(ftmesh_list, band)
-        if Filter.dtype == torch.float64:  # output dtype follows Filter (cuda_class.py:27)
-            ftmesh = ftmesh.to(torch.complex128)
-        return ftmesh
+        # output dtype follows Filter (cuda_class.py:27)
+        out_dt = torch.complex128 if Filter.dtype == torch.float64 else torch.complex64
+        return ftmesh if ftmesh.dtype == out_dt else ftmesh.to(out_dt)
 
     @staticmethod
     def backward(ctx, grad_output):
         V, Fc, C, ftmesh_list = ctx.saved_tensors
         band = ctx.band
-        grad_list = gather_band(grad_output, band)
+        grad_list = gather_band(grad_output, band, ftmesh_list.dtype)
         gradVerts = band_backward(V, Fc, C, grad_list, ftmesh_list, band)
         return gradVerts, None, None, None, None, None, None, None, None
 

@@ -108,6 +108,16 @@ def generate_gaussian_psf(sigma, xi0, xi1, xi2):
     sig_t = sigma if isinstance(sigma, torch.Tensor) else torch.as_tensor(np.asarray(sigma))
     if sig_t.numel() != 9:
         raise ValueError(f"sigma must be 3 x 3, got {tuple(sig_t.shape)}")
+    if all(isinstance(a, torch.Tensor) and a.is_cuda and a.dtype == torch.float64
+           for a in (xi0, xi1, xi2)):
+        # float64 axes (a float64 Fourier3dMesh): the reference's formula in
+        # FP64 torch ops on the GPU, separable in the axes; the HIP kernel is FP32
+        S = sig_t.to(device=xi0.device, dtype=torch.float64).reshape(3, 3)
+        a0, a1, a2 = xi0[:, None, None], xi1[None, :, None], xi2[None, None, :]
+        q = (a0 * (S[0, 0] * a0 + S[0, 1] * a1 + S[0, 2] * a2)
+             + a1 * (S[1, 0] * a0 + S[1, 1] * a1 + S[1, 2] * a2)
+             + a2 * (S[2, 0] * a0 + S[2, 1] * a1 + S[2, 2] * a2))
+        return torch.exp(-q / 2)
     axes = _gpu_axes(xi0, xi1, xi2)
     if sig_t.requires_grad and torch.is_grad_enabled():
         return _GaussianPSF.apply(sig_t, *axes)

@@ -61,6 +61,27 @@ int dmic_meshft_forward(const float* verts, int64_t nv, const void* faces, int f
                         hipStream_t stream);
 
 /*
+ * float64 boundary of dmic_meshft_forward / dmic_meshft_backward (the
+ * reference's AT_DISPATCH_FLOATING_TYPES double case,
+ * fourier3d_cuda_kernel.cu:447,508): vertices, faces_coeff, xi_list and the
+ * upstream gradient are double, ftmesh_list is complex128 (double pairs) and
+ * grad_verts double.  Inside, the same kernels run: vertex phases in FP64
+ * from the FP64 inputs, edge vectors and the face formula in FP32, FP64
+ * accumulation; the result carries the float32 path's accuracy (about 4e-7
+ * forward / 4e-6 adjoint rel-L2 against an FP64 evaluation).  Same
+ * workspaces as the float32 entry points.
+ */
+int dmic_meshft_forward_f64(const double* verts, int64_t nv, const void* faces,
+                            int faces_is_int64, int64_t nf, const double* faces_coeff,
+                            const double* xi_list, int64_t n, double* ftmesh_list,
+                            void* workspace, size_t workspace_bytes, hipStream_t stream);
+int dmic_meshft_backward_f64(const double* verts, int64_t nv, const void* faces,
+                             int faces_is_int64, int64_t nf, const double* faces_coeff,
+                             const double* grad_output_list, const double* ftmesh_list,
+                             const double* xi_list, int64_t n, double* grad_verts,
+                             void* workspace, size_t workspace_bytes, hipStream_t stream);
+
+/*
  * dmic_meshft_forward without a workspace and without any allocation.  No
  * counterpart in the reference: it is for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,

@@ -349,6 +349,57 @@ def test_closure_halving_on_fftshift_otf(device):
     assert ef <= 1e-6 and eb <= 1e-6
 
 
+def test_float64_inputs_config1(device):
+    """float64 vertices and frequencies (the reference's double dispatch,
+    fourier3d_cuda_kernel.cu:447,508): complex128 / float64 results within
+    the float32 path's tolerances of the FP64 oracle on the same FP64 inputs,
+    through the native module and through Fourier3dMesh(dtype=float64)."""
+    from deltamic_amd import Fourier3dMesh, generate_gaussian_psf
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    w = syn.config1()
+    box = torch.tensor(w.box_size)
+    mesh_ft = Fourier3dMesh(box, w.box_shape, device=device, dtype=torch.float64)
+    otf = generate_gaussian_psf(torch.tensor(w.sigma, device=device, dtype=torch.float64),
+                                mesh_ft.xi0, mesh_ft.xi1, mesh_ft.xi2)
+    assert otf.dtype == torch.float64
+    mesh_ft.OTF = otf
+    mesh_ft.narrowband_thresh = w.thresh
+    # FP64 vertices off the FP32 grid: jitter at 1e-9 below FP32 resolution
+    V64 = (torch.tensor(w.verts, dtype=torch.float64)
+           + 1e-9 * torch.tensor(syn.rng(9).normal(size=w.verts.shape)))
+    Vd = V64.to(device).requires_grad_(True)
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.ones(len(w.faces), dtype=torch.float64, device=device)
+    ft = mesh_ft(Vd, Fc, C)
+    assert ft.dtype == torch.complex128
+    from deltamic_amd.hip_class import band_for
+    band = band_for(otf, w.thresh, mesh_ft.xi0, mesh_ft.xi1, mesh_ft.xi2)
+    xi64 = band.freqs(torch.float64).cpu().numpy()
+    Vs = (V64 - mesh_ft.box_size.cpu()[:, 0]).numpy()
+    ref = O.acc_forward(Vs, w.faces, np.ones(len(w.faces)), xi64, threads=threads(),
+                        fp32_inputs=False)
+    got = ft.detach().reshape(-1)[band.idx.long()].cpu().numpy()
+    ef = O.rel_l2(got, ref)
+    g = _grad(band.n, 79).astype(np.complex128)
+    box_g = torch.zeros(band.size, dtype=torch.complex128, device=device)
+    box_g[band.idx.long()] = torch.tensor(g, device=device)
+    (gv,) = torch.autograd.grad(ft, Vd, box_g.reshape(ft.shape))
+    assert gv.dtype == torch.float64
+    refg = O.acc_backward(Vs, w.faces, np.ones(len(w.faces)), xi64, g, threads=threads(),
+                          fp32_inputs=False)
+    eb = O.rel_l2(gv.cpu().numpy(), refg)
+    # the native module's float64 entry points directly
+    Vt = torch.tensor(Vs, device=device)
+    xt = torch.tensor(xi64, device=device)
+    ft2 = forward_fourier3d_cpp(Vt, Fc, C, xt)
+    gv2 = backward_fourier3d_cpp(Vt, Fc, C, torch.tensor(g, device=device), ft2, xt)
+    ef2, eb2 = O.rel_l2(ft2.cpu().numpy(), ref), O.rel_l2(gv2.cpu().numpy(), refg)
+    print(f"float64 inputs, config 1: module forward {ef:.2e} adjoint {eb:.2e}; native forward "
+          f"{ef2:.2e} adjoint {eb2:.2e}")
+    assert max(ef, ef2) <= FWD_TOL and max(eb, eb2) <= BWD_TOL
+    assert ft2.dtype == torch.complex128 and gv2.dtype == torch.float64
+
+
 def test_full_size_cfg2_properties(device):
     """Config 2 at full size (all 1.08M band frequencies, 20,480 faces), checked
     through properties that need no oracle: Hermitian symmetry of the
