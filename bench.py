@@ -143,7 +143,7 @@ def render_leg(dev, w, band, otf, vals, reps):
     the stream it runs on, its algorithmic HBM bytes (each operand read once,
     each result written once) and the rate against 8 TB/s."""
     import torch
-    from deltamic_amd import normalize_tensor
+    from deltamic_amd import normalize_tensor, render_image_from_ftmesh
     from deltamic_amd.hip_class import gather_band, scatter_band
     nbox = int(np.prod(w.box_shape))
     n = band.idx.numel()
@@ -153,13 +153,18 @@ def render_leg(dev, w, band, otf, vals, reps):
     spec = torch.fft.ifftn(prod, shape)
     img = torch.abs(spec)
     up = torch.randn(nbox, device=dev).reshape(shape)
+    from deltamic_amd.render import _AbsHIP
+    img_h = _AbsHIP.apply(spec)
     ops = [
-        # name, fn, algorithmic bytes
+        # name, fn, algorithmic bytes (each operand read once, each result written once)
         ("scatter_band", lambda: scatter_band(vals, band), 8 * nbox + 12 * n),
         ("otf_multiply", lambda: box * otf, 20 * nbox),
         ("ifftn", lambda: torch.fft.ifftn(prod, shape), 16 * nbox),
-        ("abs", lambda: torch.abs(spec), 12 * nbox),
-        ("normalize", lambda: normalize_tensor(img), 16 * nbox),
+        ("abs_minmax", lambda: _AbsHIP.apply(spec), 12 * nbox),
+        ("normalize", lambda: normalize_tensor(img_h), 16 * nbox),
+        ("torch_abs", lambda: torch.abs(spec), 12 * nbox),
+        ("torch_normalize", lambda: ((img - img.min()) / (img.max() - img.min()) + 0) / 1,
+         16 * nbox),
         ("gather_band", lambda: gather_band(prod, band), 12 * n),
     ]
     res = {}
@@ -180,20 +185,28 @@ def render_leg(dev, w, band, otf, vals, reps):
     boxg = box.clone().requires_grad_(True)
 
     def chain_bwd():
-        out = normalize_tensor(torch.abs(torch.fft.ifftn(boxg * otf, shape)))
+        out = normalize_tensor(render_image_from_ftmesh(boxg, otf, shape))
         (gb,) = torch.autograd.grad(out, boxg, up)
         return gb
 
-    chain_bwd()
-    torch.cuda.synchronize()
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(reps):
-        chain_bwd()
-    b.record()
-    torch.cuda.synchronize()
-    res["chain_forward_and_backward_ms"] = round(a.elapsed_time(b) / reps, 4)
+    def chain_bwd_torch():
+        t = torch.abs(torch.fft.ifftn(boxg * otf, shape))
+        out = (t - t.min()) / (t.max() - t.min())
+        (gb,) = torch.autograd.grad(out, boxg, up)
+        return gb
+
+    for key, fn in (("chain_forward_and_backward_ms", chain_bwd),
+                    ("torch_chain_forward_and_backward_ms", chain_bwd_torch)):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        res[key] = round(a.elapsed_time(b) / reps, 4)
     res["box"] = list(w.box_shape)
     res["band"] = n
     res["hbm_peak_GBps"] = 8000.0
@@ -517,7 +530,8 @@ def main() -> int:
         render = render_leg(dev, w, band, otf, out, args.render_reps)
         log("render chain on " + str(w.box_shape) + ": " + ", ".join(
             f"{k} {v['ms']:.4f} ms ({v['GBps']:.0f} GB/s)" for k, v in render.items()
-            if isinstance(v, dict)) + f"; chain fwd+bwd {render['chain_forward_and_backward_ms']} ms")
+            if isinstance(v, dict)) + f"; chain fwd+bwd {render['chain_forward_and_backward_ms']} ms "
+            f"(torch ops: {render['torch_chain_forward_and_backward_ms']} ms)")
 
     psf = None
     if args.psf_steps > 0:

@@ -64,6 +64,12 @@ SIGNATURES = {
     "dmic_laplacian_solve_workspace": ([_i64], _sz),
     "dmic_laplacian_solve": ([_vp, _vp, _i64, _f32, _int, _vp, _vp, _f32, _int, _vp, _vp, _sz,
                               _vp], _int),
+    "dmic_render_workspace": ([_i64], _sz),
+    "dmic_abs_minmax": ([_vp, _i64, _vp, _vp, _vp, _sz, _vp], _int),
+    "dmic_minmax": ([_vp, _i64, _vp, _vp, _sz, _vp], _int),
+    "dmic_normalize": ([_vp, _i64, _vp, _f32, _vp, _vp], _int),
+    "dmic_normalize_backward": ([_vp, _vp, _i64, _vp, _f32, _vp, _vp, _sz, _vp], _int),
+    "dmic_abs_backward": ([_vp, _vp, _i64, _vp, _vp], _int),
     "dmic_gemm_bf16": ([_vp, _int, _i64, _i64, _vp, _int, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
                         _vp, _vp, _i64, _int, _vp], _int),
     "dmic_mlp_weight_count": ([], _i64),

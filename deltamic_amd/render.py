@@ -13,11 +13,90 @@ import numpy as np
 import torch
 from torch import nn
 
+from . import _lib
 from .hip_class import MeshFTAutofluorescenceHIP, MeshFTHIP, clear_band_cache
 
 
+class _NormalizeHIP(torch.autograd.Function):
+    """normalize_tensor on a CUDA float32 tensor with the HIP kernels of
+    csrc/render.hip: a min/max pass (skipped when the tensor comes from
+    render_image_from_ftmesh, whose abs pass already reduced it) and one
+    elementwise pass; backward is one reduction and one elementwise pass,
+    with torch's gradient through min and max (shared evenly by ties)."""
+
+    @staticmethod
+    def forward(ctx, t, c):
+        L = _lib.lib()
+        st = _lib.stream_of(t)
+        ws = _lib.workspace(L.dmic_render_workspace(t.numel()), t.device)
+        cached = getattr(t, "_dmic_minmax", None)
+        if cached is not None and cached[1] == t._version:
+            mm = cached[0]
+        else:
+            mm = torch.empty(2, dtype=torch.float32, device=t.device)
+            _lib.check(L.dmic_minmax(t.data_ptr(), t.numel(), mm.data_ptr(), ws.data_ptr(),
+                                     ws.numel(), st), "dmic_minmax")
+        out = torch.empty_like(t)
+        _lib.check(L.dmic_normalize(t.data_ptr(), t.numel(), mm.data_ptr(), c, out.data_ptr(), st),
+                   "dmic_normalize")
+        ctx.save_for_backward(t, mm)
+        ctx.c = c
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        t, mm = ctx.saved_tensors
+        g = g.contiguous().to(torch.float32)
+        L = _lib.lib()
+        ws = _lib.workspace(L.dmic_render_workspace(t.numel()), t.device)
+        grad_t = torch.empty_like(t)
+        _lib.check(L.dmic_normalize_backward(g.data_ptr(), t.data_ptr(), t.numel(), mm.data_ptr(),
+                                             ctx.c, grad_t.data_ptr(), ws.data_ptr(), ws.numel(),
+                                             _lib.stream_of(t)), "dmic_normalize_backward")
+        return grad_t, None
+
+
+class _AbsHIP(torch.autograd.Function):
+    """|z| of a CUDA complex64 box (HIP kernel; its min and max come out of
+    the same pass and are kept on the result for normalize_tensor);
+    backward g z / |z| (torch's sgn, 0 at z = 0)."""
+
+    @staticmethod
+    def forward(ctx, z):
+        L = _lib.lib()
+        st = _lib.stream_of(z)
+        img = torch.empty(z.shape, dtype=torch.float32, device=z.device)
+        mm = torch.empty(2, dtype=torch.float32, device=z.device)
+        ws = _lib.workspace(L.dmic_render_workspace(z.numel()), z.device)
+        _lib.check(L.dmic_abs_minmax(z.data_ptr(), z.numel(), img.data_ptr(), mm.data_ptr(),
+                                     ws.data_ptr(), ws.numel(), st), "dmic_abs_minmax")
+        ctx.save_for_backward(z)
+        img._dmic_mm = mm
+        return img
+
+    @staticmethod
+    def backward(ctx, g):
+        (z,) = ctx.saved_tensors
+        g = g.contiguous().to(torch.float32)
+        gz = torch.empty_like(z)
+        L = _lib.lib()
+        _lib.check(L.dmic_abs_backward(g.data_ptr(), z.data_ptr(), z.numel(), gz.data_ptr(),
+                                       _lib.stream_of(z)), "dmic_abs_backward")
+        return gz
+
+
+def _hip_ok(t, dtype) -> bool:
+    return isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.numel() > 0
+
+
 def normalize_tensor(t, contrast_variable=0):
-    """((t - min)/(max - min) + c)/(1 + c), gradient through min and max (render.py:5-6)."""
+    """((t - min)/(max - min) + c)/(1 + c), gradient through min and max (render.py:5-6).
+
+    CUDA float32 tensors with a numeric ``contrast_variable`` use the HIP
+    kernels (same FP32 operation order, so the forward values match torch's
+    formula); other tensors use the formula in torch."""
+    if _hip_ok(t, torch.float32) and isinstance(contrast_variable, (int, float)):
+        return _NormalizeHIP.apply(t.contiguous(), float(contrast_variable))
     return ((t - t.min()) / (t.max() - t.min()) + contrast_variable) / (1 + contrast_variable)
 
 
@@ -109,7 +188,15 @@ def render_image_from_ftmesh(ftmesh, OTF, box_shape):
     reference (the centred grid's linear phase disappears in the modulus).
     """
     ftexp = ftmesh * OTF
-    return torch.abs(torch.fft.ifftn(ftexp, list(box_shape)))
+    spec = torch.fft.ifftn(ftexp, list(box_shape))
+    if _hip_ok(spec, torch.complex64):
+        spec = spec.contiguous()
+        img = _AbsHIP.apply(spec)
+        # the abs pass also reduced min and max; normalize_tensor reuses them
+        # while the image is unchanged (same version)
+        img._dmic_minmax = (img._dmic_mm, img._version)
+        return img
+    return torch.abs(spec)
 
 
 def compute_spatial_frequency_grid(box_shape, box_size, device='cpu', dtype=torch.float):

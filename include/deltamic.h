@@ -263,6 +263,34 @@ int dmic_psf_aniso_grad(const float* prm, const float* grad_out, const float* ps
                         int64_t n2, float* grad_prm, void* workspace, size_t workspace_bytes,
                         hipStream_t stream);
 
+/* ------------------------------------------------- render chain
+ *
+ * The modulus and normalisation after the inverse FFT of
+ * render_image_from_ftmesh (deltamic/render.py:177-180) and normalize_tensor
+ * (render.py:5-6), forward and backward, as HBM-bound passes.  z is a
+ * complex64 box (float pairs), t / img / out float; minmax is a device
+ * float[2] (min, max); NaN propagates through min and max.  The workspace
+ * (dmic_render_workspace) holds fixed-order reduction partials.
+ */
+size_t dmic_render_workspace(int64_t n);
+/* img = |z| (hypot) and minmax = (min img, max img) in the same pass. */
+int dmic_abs_minmax(const float* z, int64_t n, float* img, float* minmax, void* workspace,
+                    size_t workspace_bytes, hipStream_t stream);
+/* minmax = (min t, max t). */
+int dmic_minmax(const float* t, int64_t n, float* minmax, void* workspace, size_t workspace_bytes,
+                hipStream_t stream);
+/* out = ((t - min) / (max - min) + c) / (1 + c) in FP32, torch's operation order. */
+int dmic_normalize(const float* t, int64_t n, const float* minmax, float c, float* out,
+                   hipStream_t stream);
+/* grad_t = dL/dt of normalize given g = dL/dout, including the paths through
+ * min and max (each shared evenly by the entries that attain it, as torch). */
+int dmic_normalize_backward(const float* g, const float* t, int64_t n, const float* minmax,
+                            float c, float* grad_t, void* workspace, size_t workspace_bytes,
+                            hipStream_t stream);
+/* grad_z = g z / |z| (0 where z = 0): the backward of |z|. */
+int dmic_abs_backward(const float* g, const float* z, int64_t n, float* grad_z,
+                      hipStream_t stream);
+
 /* ------------------------------------------------- large-steps preconditioner
  *
  * x = (I + lambda L)^-power g for the vertex gradient g (float [nv][3]),

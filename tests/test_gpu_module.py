@@ -206,3 +206,45 @@ def test_wrong_layout_and_dtype_raise(device):
         forward_fourier3d_cpp(V, Fc, C, xi)
     with pytest.raises(TypeError):
         forward_fourier3d_cpp(V.double(), Fc, C, xi.contiguous())
+
+
+def test_render_chain_hip_matches_torch(device):
+    """abs + normalize_tensor on the HIP kernels against torch's own ops on
+    the same complex box: forward values equal to 1 ulp, gradient w.r.t. the
+    box (through ifftn, abs, min/max and the normalisation) to 1e-6, with
+    ties at the minimum and maximum sharing their gradient as in torch."""
+    from deltamic_amd import normalize_tensor, render_image_from_ftmesh
+    g = torch.Generator(device="cpu").manual_seed(5)
+    shape = (48, 40, 36)
+    ft = torch.complex(torch.randn(shape, generator=g), torch.randn(shape, generator=g))
+    otf = torch.rand(shape, generator=g)
+    up = torch.randn(shape, generator=g)
+    def torch_chain(box, o, c):
+        t = torch.abs(torch.fft.ifftn(box * o, list(shape)))
+        return ((t - t.min()) / (t.max() - t.min()) + c) / (1 + c)
+
+    for c in (0, 0.2):
+        a = ft.to(device).requires_grad_(True)
+        img = normalize_tensor(render_image_from_ftmesh(a, otf.to(device), shape), c)
+        (ga,) = torch.autograd.grad(img, a, up.to(device))
+        # torch's ops on the same FP32 box, and in FP64 (the FFT's own rounding)
+        b = ft.to(device).requires_grad_(True)
+        ref = torch_chain(b, otf.to(device), c)
+        (gb,) = torch.autograd.grad(ref, b, up.to(device))
+        d = ft.to(torch.complex128).to(device).requires_grad_(True)
+        ref64 = torch_chain(d, otf.double().to(device), c)
+        (gd,) = torch.autograd.grad(ref64, d, up.double().to(device))
+        ef = (img - ref).abs().max().item()
+        eb = ((ga - gb).abs().norm() / gb.abs().norm()).item()
+        eb64 = ((ga.to(torch.complex128) - gd).abs().norm() / gd.abs().norm()).item()
+        tb64 = ((gb.to(torch.complex128) - gd).abs().norm() / gd.abs().norm()).item()
+        print(f"render chain c={c}: forward max abs diff vs torch {ef:.2e}; box gradient rel-L2 "
+              f"vs torch FP32 {eb:.2e}, vs FP64 {eb64:.2e} (torch FP32 vs FP64 {tb64:.2e})")
+        assert ef <= 1e-6 and eb <= 1e-6 and eb64 <= 2 * tb64 + 1e-7
+    # ties: the minimum (0) and maximum (1) each attained several times
+    t = torch.tensor([0.0, 1.0, 0.5, 0.0, 1.0, 0.25, 0.0], device=device, requires_grad=True)
+    u = torch.tensor([1.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0], device=device)
+    (gh,) = torch.autograd.grad(normalize_tensor(t, 0.3), t, u)
+    tc = t.detach().cpu().requires_grad_(True)
+    (gr,) = torch.autograd.grad(((tc - tc.min()) / (tc.max() - tc.min()) + 0.3) / 1.3, tc, u.cpu())
+    assert torch.allclose(gh.cpu(), gr, rtol=1e-6, atol=1e-7)
