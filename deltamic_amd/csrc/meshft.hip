@@ -87,37 +87,64 @@ __device__ __forceinline__ void load_face_verts(const T* __restrict__ V, int64_t
   }
 }
 
-// c_t A_t (FP64, cross-product norm as .cu:9-12,38-40) and, in one
-// workgroup and a fixed order, S[0] = sum_t c_t A_t and
-// S[1] = regvol = -sum_t a . (b x c) / 6 (compute_volume_manifold,
-// render.py:221-226; used by the autofluorescence variant).
+// c_t A_t (FP64, cross-product norm as .cu:9-12,38-40) per face and, per
+// workgroup, the partial sums of c_t A_t and of a . (b x c) (for
+// regvol = -sum_t a . (b x c) / 6, compute_volume_manifold, render.py:221-226;
+// used by the autofluorescence variant).  face_sum_final_kernel adds the
+// partials in a fixed tree: S[0] = sum_t c_t A_t, S[1] = regvol.
 template <typename IdxT, typename T>
-__global__ __launch_bounds__(1024) void face_area_sum_kernel(
+__global__ __launch_bounds__(256) void face_area_sum_kernel(
     const T* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
-    const T* __restrict__ coeff, double* __restrict__ cA, double* __restrict__ S_out,
+    const T* __restrict__ coeff, double* __restrict__ cA, double* __restrict__ part,
     int* flag) {
-  __shared__ double red[1024];
-  __shared__ double redv[1024];
+  __shared__ double red[256];
+  __shared__ double redv[256];
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   double s = 0.0, vs = 0.0;
-  for (int64_t t = threadIdx.x; t < nf; t += 1024) {
+  if (t < nf) {
     double P[3][3];
     load_face_verts(V, nv, Fc, t, P, flag);
     double u[3], q[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      u[d] = (double)P[1][d] - (double)P[0][d];
-      q[d] = (double)P[2][d] - (double)P[0][d];
+      u[d] = P[1][d] - P[0][d];
+      q[d] = P[2][d] - P[0][d];
     }
     const double n0 = u[1] * q[2] - u[2] * q[1];
     const double n1 = u[2] * q[0] - u[0] * q[2];
     const double n2 = u[0] * q[1] - u[1] * q[0];
     const double ca = (double)coeff[t] * 0.5 * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
     cA[t] = ca;
-    s += ca;
-    const double p0 = P[0][0], p1 = P[0][1], p2 = P[0][2];
-    vs += p0 * ((double)P[1][1] * P[2][2] - (double)P[1][2] * P[2][1]) +
-          p1 * ((double)P[1][2] * P[2][0] - (double)P[1][0] * P[2][2]) +
-          p2 * ((double)P[1][0] * P[2][1] - (double)P[1][1] * P[2][0]);
+    s = ca;
+    vs = P[0][0] * (P[1][1] * P[2][2] - P[1][2] * P[2][1]) +
+         P[0][1] * (P[1][2] * P[2][0] - P[1][0] * P[2][2]) +
+         P[0][2] * (P[1][0] * P[2][1] - P[1][1] * P[2][0]);
+  }
+  red[threadIdx.x] = s;
+  redv[threadIdx.x] = vs;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      red[threadIdx.x] += red[threadIdx.x + w];
+      redv[threadIdx.x] += redv[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = red[0];
+    part[2 * blockIdx.x + 1] = redv[0];
+  }
+}
+
+__global__ __launch_bounds__(1024) void face_sum_final_kernel(const double* __restrict__ part,
+                                                              int64_t nblocks,
+                                                              double* __restrict__ S_out) {
+  __shared__ double red[1024];
+  __shared__ double redv[1024];
+  double s = 0.0, vs = 0.0;
+  for (int64_t b = threadIdx.x; b < nblocks; b += 1024) {
+    s += part[2 * b];
+    vs += part[2 * b + 1];
   }
   red[threadIdx.x] = s;
   redv[threadIdx.x] = vs;
@@ -573,7 +600,9 @@ __global__ __launch_bounds__(256) void csr_count_kernel(const IdxT* __restrict__
   if (v >= 0 && v < nv) atomicAdd(&cnt[v], 1);  // integer: order-independent
 }
 
-// Exclusive scan of cnt[0..count) into off[0..count], off[count] = total, in one workgroup.
+// Exclusive scan of cnt[0..count) into off[0..count], off[count] = total, in
+// one workgroup: each thread sums a contiguous range, the 1024 range sums
+// are scanned in LDS (Hillis-Steele), then each thread writes its range.
 __global__ __launch_bounds__(1024) void scan_i32_kernel(const int* __restrict__ cnt, int64_t count,
                                                         int* __restrict__ off) {
   __shared__ int64_t sums[1024];
@@ -584,17 +613,14 @@ __global__ __launch_bounds__(1024) void scan_i32_kernel(const int* __restrict__ 
   for (int64_t i = b; i < e; ++i) s += cnt[i];
   sums[threadIdx.x] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int64_t run = 0;
-    for (int i = 0; i < 1024; ++i) {
-      const int64_t v = sums[i];
-      sums[i] = run;
-      run += v;
-    }
-    off[count] = (int)run;
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int64_t add = ((int)threadIdx.x >= d) ? sums[threadIdx.x - d] : 0;
+    __syncthreads();
+    sums[threadIdx.x] += add;
+    __syncthreads();
   }
-  __syncthreads();
-  int64_t run = sums[threadIdx.x];
+  int64_t run = sums[threadIdx.x] - s;  // exclusive prefix of this thread's range
+  if (threadIdx.x == 1023) off[count] = (int)sums[1023];
   for (int64_t i = b; i < e; ++i) {
     off[i] = (int)run;
     run += cnt[i];
@@ -750,6 +776,7 @@ struct FwdWs {
   int* flag;
   double* cA;
   double* S;
+  double* spart;  // per-workgroup partials of the area and volume sums
   FaceRec* recs;
   double2* part;
 };
@@ -759,6 +786,7 @@ static FwdWs carve_fwd(Carver& c, int64_t nf, int64_t n, int64_t part_cap = -1) 
   w.flag = c.take<int>(4);
   w.cA = c.take<double>(nf > 0 ? nf : 1);
   w.S = c.take<double>(2);
+  w.spart = c.take<double>(2 * (size_t)ceil_div(nf > 0 ? nf : 1, 256));
   w.recs = c.take<FaceRec>(nf > 0 ? nf : 1);
   const FwdPlan p = fwd_plan(nf, n, part_cap);
   w.part = (p.nsplit > 1) ? c.take<double2>((size_t)p.nsplit * n) : nullptr;
@@ -769,6 +797,7 @@ struct BwdWs {
   int* flag;
   double* cA;
   double* S;
+  double* spart;
   FaceRec* recs;
   double* part;
   double* G;
@@ -790,6 +819,7 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n, bool af) {
   w.flag = c.take<int>(4);
   w.cA = c.take<double>(f1);
   w.S = c.take<double>(2);
+  w.spart = c.take<double>(2 * (size_t)ceil_div(f1, 256));
   w.recs = c.take<FaceRec>(f1);
   const BwdPlan p = bwd_plan(nf, n);
   w.part = c.take<double>((size_t)p.nchunk * (af ? kFaceSums<true> : kFaceSums<false>) * f1);
@@ -809,10 +839,13 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n, bool af) {
 
 template <typename IdxT, typename T>
 static int prep_faces(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const T* coeff,
-                      float regint, int* flag, double* cA, double* S, FaceRec* recs,
-                      hipStream_t st) {
-  hipLaunchKernelGGL((face_area_sum_kernel<IdxT, T>), dim3(1), dim3(1024), 0, st, V, nv, Fc, nf,
-                     coeff, cA, S, flag);
+                      float regint, int* flag, double* cA, double* S, double* spart,
+                      FaceRec* recs, hipStream_t st) {
+  const unsigned nb = ceil_div(nf, 256);
+  hipLaunchKernelGGL((face_area_sum_kernel<IdxT, T>), dim3(nb), dim3(256), 0, st, V, nv, Fc, nf,
+                     coeff, cA, spart, flag);
+  DMIC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(face_sum_final_kernel, dim3(1), dim3(1024), 0, st, spart, (int64_t)nb, S);
   DMIC_LAUNCH_CHECK();
   hipLaunchKernelGGL((face_pack_kernel<IdxT, T>), dim3(ceil_div(nf, 256)), dim3(256), 0, st, V, nv,
                      Fc, nf, cA, S, regint, recs, flag);
@@ -840,7 +873,7 @@ static int forward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, cons
     return DMIC_OK;
   }
   int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S,
-                               w.recs, st);
+                               w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
   const FwdPlan p = fwd_plan(nf, n, part_cap);
   dim3 grid(ceil_div(n, 256), p.nsplit);
@@ -879,7 +912,7 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
     return DMIC_OK;
   }
   int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S,
-                               w.recs, st);
+                               w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
   const BwdPlan p = bwd_plan(nf, n);
   if (n > 0) {

@@ -11,9 +11,8 @@
 //   backward: one reduction pass over (grad, t) for the min/max terms and
 //             one elementwise pass for dL/dt; abs backward reads the complex
 //             box once more (dL/dz = g z / |z|, 0 at z = 0, torch's sgn).
-// All HBM-bound, grid-stride loops with 16-byte loads where the layout
-// allows.  Reductions: per-block partials in a fixed order, then one block
-// adds them in order (bitwise reproducible).  NaN propagates through min and
+// All HBM-bound grid-stride loops.  Reductions: per-block partials, then
+// one block reduces them in a fixed tree (bitwise reproducible).  NaN propagates through min and
 // max as in torch.  Ties at the minimum or maximum share its gradient
 // evenly (torch's evenly_distribute_backward for full reductions).
 #include "common.h"
@@ -88,17 +87,33 @@ __global__ __launch_bounds__(kRThreads) void minmax_kernel(const float* __restri
   }
 }
 
-// minmax[0..1] = (min, max) over the block partials, in block order.
-__global__ __launch_bounds__(64) void minmax_final_kernel(const float* __restrict__ part,
-                                                          int nblocks, float* __restrict__ minmax) {
-  if (threadIdx.x != 0) return;
-  float mn = part[0], mx = part[1];
-  for (int b = 1; b < nblocks; ++b) {
+constexpr int kFinal = 1024;  // threads of the one-block final reductions
+
+// minmax[0..1] = (min, max) over the block partials: one block, each thread
+// a fixed strided subset, then a fixed tree.
+__global__ __launch_bounds__(kFinal) void minmax_final_kernel(const float* __restrict__ part,
+                                                              int nblocks,
+                                                              float* __restrict__ minmax) {
+  __shared__ float smn[kFinal], smx[kFinal];
+  float mn = __builtin_inff(), mx = -__builtin_inff();
+  for (int b = threadIdx.x; b < nblocks; b += kFinal) {
     mn = nan_min(mn, part[2 * b]);
     mx = nan_max(mx, part[2 * b + 1]);
   }
-  minmax[0] = mn;
-  minmax[1] = mx;
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = kFinal / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      smn[threadIdx.x] = nan_min(smn[threadIdx.x], smn[threadIdx.x + w]);
+      smx[threadIdx.x] = nan_max(smx[threadIdx.x], smx[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    minmax[0] = smn[0];
+    minmax[1] = smx[0];
+  }
 }
 
 // out = ((t - min) / (max - min) + c) / (1 + c), FP32 in torch's operation order.
@@ -145,19 +160,22 @@ __global__ __launch_bounds__(kRThreads) void normalize_bwd_reduce_kernel(
 
 // coef = {r, dmin / count_min, dmax / count_max} with r = 1/((max - min)(1 + c)),
 // dmin = r sum g (u - 1), dmax = -r sum g u.
-__global__ __launch_bounds__(64) void normalize_bwd_final_kernel(const double* __restrict__ part,
-                                                                 int nblocks,
-                                                                 const float* __restrict__ minmax,
-                                                                 float c,
-                                                                 double* __restrict__ coef) {
-  if (threadIdx.x != 0) return;
-  double s1 = 0.0, s2 = 0.0, cmin = 0.0, cmax = 0.0;
-  for (int b = 0; b < nblocks; ++b) {
-    s1 += part[4 * b];
-    s2 += part[4 * b + 1];
-    cmin += part[4 * b + 2];
-    cmax += part[4 * b + 3];
+__global__ __launch_bounds__(kFinal) void normalize_bwd_final_kernel(
+    const double* __restrict__ part, int nblocks, const float* __restrict__ minmax, float c,
+    double* __restrict__ coef) {
+  __shared__ double sh[4][kFinal];
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblocks; b += kFinal)
+    for (int k = 0; k < 4; ++k) a[k] += part[4 * b + k];
+  for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = a[k];
+  __syncthreads();
+  for (int w = kFinal / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
   }
+  if (threadIdx.x != 0) return;
+  const double s1 = sh[0][0], s2 = sh[1][0], cmin = sh[2][0], cmax = sh[3][0];
   const double r = 1.0 / (((double)minmax[1] - (double)minmax[0]) * (1.0 + (double)c));
   coef[0] = r;
   coef[1] = (cmin > 0.0) ? r * (s2 - s1) / cmin : 0.0;
@@ -225,7 +243,7 @@ int dmic_abs_minmax(const float* z, int64_t n, float* img, float* minmax, void* 
   hipLaunchKernelGGL(abs_minmax_kernel, dim3(nb), dim3(kRThreads), 0, stream,
                      reinterpret_cast<const float2*>(z), n, img, part);
   DMIC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(64), 0, stream, part, (int)nb, minmax);
+  hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(kFinal), 0, stream, part, (int)nb, minmax);
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
 }
@@ -240,7 +258,7 @@ int dmic_minmax(const float* t, int64_t n, float* minmax, void* workspace, size_
   const unsigned nb = render_blocks(n);
   hipLaunchKernelGGL(minmax_kernel, dim3(nb), dim3(kRThreads), 0, stream, t, n, part);
   DMIC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(64), 0, stream, part, (int)nb, minmax);
+  hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(kFinal), 0, stream, part, (int)nb, minmax);
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
 }
@@ -269,7 +287,7 @@ int dmic_normalize_backward(const float* g, const float* t, int64_t n, const flo
   hipLaunchKernelGGL(normalize_bwd_reduce_kernel, dim3(nb), dim3(kRThreads), 0, stream, g, t, n,
                      minmax, part);
   DMIC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(normalize_bwd_final_kernel, dim3(1), dim3(64), 0, stream, part, (int)nb,
+  hipLaunchKernelGGL(normalize_bwd_final_kernel, dim3(1), dim3(kFinal), 0, stream, part, (int)nb,
                      minmax, c, coef);
   DMIC_LAUNCH_CHECK();
   hipLaunchKernelGGL(normalize_bwd_kernel, dim3(nb), dim3(kRThreads), 0, stream, g, t, n, minmax,
