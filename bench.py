@@ -274,7 +274,7 @@ def learned_fit_leg(dev, steps):
     import torch
     from deltamic_amd import synthetic as syn
     from deltamic_amd.fit import fit_adam, make_problem
-    from deltamic_amd.proposal import Features, ProposalMLP, fit_learned
+    from deltamic_amd.proposal import N_IN, Features, ProposalMLP, fit_learned
     net = ProposalMLP(dev)
     net.load_state_dict(torch.load(os.path.join(ROOT, "deltamic_amd", "weights", "proposal.pt"),
                                    weights_only=True))
@@ -297,11 +297,27 @@ def learned_fit_leg(dev, steps):
                "learned_iterations": len(learned.losses), "learned_accepted": learned.accepted,
                "learned_s": round(t_l, 3), "reached": bool(reached),
                "speedup": round(adam.times[first] / t_l, 3)}
+        # the proposal network alone: forward (keeping activations) + backward
+        # over one row per vertex, 20 repetitions between HIP events
+        nv = prob.init_verts.shape[0]
+        x = torch.randn(nv, N_IN, device=dev, generator=torch.Generator(dev).manual_seed(5))
+        y, acts = net.forward(x, keep_acts=True)
+        net.backward(x, acts, y)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(20):
+            y, acts = net.forward(x, keep_acts=True)
+            net.backward(x, acts, y)
+        b.record()
+        torch.cuda.synchronize()
+        rec["mlp_forward_backward_ms"] = round(a.elapsed_time(b) / 20, 4)
+        rec["mlp_rows"] = nv
         res["shapes"].append(rec)
         log(f"learned fit, {len(w.faces)} faces (seed {w.seed}): plain Adam first reaches its "
             f"{steps}-step loss {target:.6f} at step {first} ({adam.times[first]:.2f} s); learned "
             f"fit {len(learned.losses)} iterations {t_l:.2f} s ({'reached' if reached else 'MISSED'}"
-            f"): {rec['speedup']:.2f}x")
+            f"): {rec['speedup']:.2f}x; proposal MLP forward+backward over {nv} rows "
+            f"{rec['mlp_forward_backward_ms']} ms")
         del prob, feats
     res["min_speedup"] = min(r["speedup"] for r in res["shapes"])
     return res

@@ -4,8 +4,9 @@
 //
 // No counterpart in the reference: the learned mesh-displacement proposal
 // is new in this build.  Every GEMM has one wave per 16x16 output tile that
-// walks K in order, so weight gradients (K = number of vertices) are
-// reduced in a fixed order without atomics.
+// walks K in order; the weight gradients (K = number of vertices, few output
+// tiles) split K into 256-row chunks whose partial tiles are added in chunk
+// order, so every result is reduced in a fixed order without atomics.
 #include "common.h"
 
 namespace dmic {
@@ -77,15 +78,91 @@ __global__ __launch_bounds__(64) void gemm_bf16_kernel(const void* __restrict__ 
   }
 }
 
-// out[j] = sum_r X[r*ld + j] in row order (one thread per column).
-__global__ __launch_bounds__(128) void colsum_kernel(const float* __restrict__ X, int64_t rows,
-                                                     int64_t cols, int64_t ld,
-                                                     float* __restrict__ out) {
+// Split-K partial of C = A B for the weight gradients (K = rows, large;
+// few output tiles): wave (x, y, z) computes the 16x16 tile (x, y) over
+// k in [z kchunk, (z + 1) kchunk) into P[z][M][N]; splitk_reduce_kernel adds
+// the partials in z order.  Same operand layout as gemm_bf16_kernel.
+template <bool ABF, bool BBF>
+__global__ __launch_bounds__(64) void gemm_bf16_splitk_kernel(
+    const void* __restrict__ A, int64_t sa0, int64_t sa1, const void* __restrict__ B, int64_t sb0,
+    int64_t sb1, float* __restrict__ P, int64_t M, int64_t N, int64_t K, int64_t kchunk) {
+  const int l = threadIdx.x;
+  const int64_t m0 = (int64_t)blockIdx.x * 16, n0 = (int64_t)blockIdx.y * 16;
+  const int64_t kb = (int64_t)blockIdx.z * kchunk;
+  const int64_t ke = (kb + kchunk < K) ? kb + kchunk : K;
+  const int64_t ar = m0 + (l & 15);
+  const int64_t bc = n0 + (l & 15);
+  const int kq = 4 * (l >> 4);
+  floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int64_t k0 = kb; k0 < ke; k0 += 16) {
+    bf16x4 a, b;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int64_t k = k0 + kq + e;
+      const float av = (ar < M && k < ke) ? load_elem<ABF>(A, ar * sa0 + k * sa1) : 0.0f;
+      const float bv = (bc < N && k < ke) ? load_elem<BBF>(B, k * sb0 + bc * sb1) : 0.0f;
+      a[e] = f32_to_bf16(av);
+      b[e] = f32_to_bf16(bv);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  float* Pz = P + (int64_t)blockIdx.z * M * N;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int64_t r = m0 + kq + e, c = bc;
+    if (r < M && c < N) Pz[r * N + c] = acc[e];
+  }
+}
+
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restrict__ P, int nz,
+                                                            int64_t M, int64_t N,
+                                                            float* __restrict__ C, int64_t ldc) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= M * N) return;
+  float s = 0.0f;
+  for (int z = 0; z < nz; ++z) s += P[(int64_t)z * M * N + i];
+  C[(i / N) * ldc + i % N] = s;
+}
+
+// Column sums out[j] = sum_r X[r*ld + j] in two fixed-order stages.
+// colsum_part_kernel: block (x, y) covers 32 columns and rows
+// [y kColChunk, (y + 1) kColChunk): 8 row lanes per column (coalesced rows)
+// each sum every 8th row in order, then the 8 lanes are added in order into
+// part[y][j] (or straight into out[j] when there is one chunk);
+// colsum_final_kernel adds the chunks in order.
+constexpr int64_t kColChunk = 256;
+__global__ __launch_bounds__(256) void colsum_part_kernel(const float* __restrict__ X,
+                                                          int64_t rows, int64_t cols, int64_t ld,
+                                                          double* __restrict__ part,
+                                                          float* __restrict__ out) {
+  __shared__ double sh[8][32];
+  const int lane = threadIdx.x & 31, rl = threadIdx.x >> 5;
+  const int64_t j = (int64_t)blockIdx.x * 32 + lane;
+  const int64_t rb = (int64_t)blockIdx.y * kColChunk;
+  const int64_t re = (rb + kColChunk < rows) ? rb + kColChunk : rows;
+  double s = 0.0;
+  if (j < cols)
+    for (int64_t r = rb + rl; r < re; r += 8) s += X[r * ld + j];
+  sh[rl][lane] = s;
+  __syncthreads();
+  if (rl == 0 && j < cols) {
+    double t = 0.0;
+    for (int k = 0; k < 8; ++k) t += sh[k][lane];
+    if (out)
+      out[j] = (float)t;
+    else
+      part[(int64_t)blockIdx.y * cols + j] = t;
+  }
+}
+
+__global__ __launch_bounds__(128) void colsum_final_kernel(const double* __restrict__ part,
+                                                           int64_t nchunk, int64_t cols,
+                                                           float* __restrict__ out) {
   const int64_t j = (int64_t)blockIdx.x * 128 + threadIdx.x;
   if (j >= cols) return;
-  double s = 0.0;
-  for (int64_t r = 0; r < rows; ++r) s += X[r * ld + j];
-  out[j] = (float)s;
+  double t = 0.0;
+  for (int64_t c = 0; c < nchunk; ++c) t += part[c * cols + j];
+  out[j] = (float)t;
 }
 
 __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ w,
@@ -135,10 +212,51 @@ static int gemm(const void* A, int64_t sa0, int64_t sa1, const void* B, int64_t 
   return DMIC_OK;
 }
 
+// Weight gradient C[M,N] = A B over K = rows with split K when the output
+// has few tiles and K is long; part: at least ceil(K / kSplitChunk) M N
+// floats of free scratch.
+constexpr int64_t kSplitChunk = 256;
+template <bool ABF, bool BBF>
+static int gemm_wgrad(const void* A, int64_t sa0, int64_t sa1, const void* B, int64_t sb0,
+                      int64_t sb1, float* C, int64_t ldc, int64_t M, int64_t N, int64_t K,
+                      float* part, hipStream_t st) {
+  const int64_t tiles = ((M + 15) / 16) * ((N + 15) / 16);
+  if (K < 4 * kSplitChunk || tiles >= 1024 || part == nullptr)
+    return gemm<ABF, BBF>(A, sa0, sa1, B, sb0, sb1, C, ldc, M, N, K, nullptr, nullptr, 0,
+                          EPI_NONE, st);
+  const int64_t nz = (K + kSplitChunk - 1) / kSplitChunk;
+  DMIC_REQUIRE(nz <= 65535, DMIC_ERR_ARG, "too many split-K chunks");
+  dim3 grid((unsigned)((M + 15) / 16), (unsigned)((N + 15) / 16), (unsigned)nz);
+  hipLaunchKernelGGL((gemm_bf16_splitk_kernel<ABF, BBF>), grid, dim3(64), 0, st, A, sa0, sa1, B,
+                     sb0, sb1, part, M, N, K, kSplitChunk);
+  DMIC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3(ceil_div(M * N, 256)), dim3(256), 0, st, part,
+                     (int)nz, M, N, C, ldc);
+  DMIC_LAUNCH_CHECK();
+  return DMIC_OK;
+}
+
+// part: free scratch of ceil(rows / kColChunk) * cols doubles, used only
+// when rows > kColChunk (then at most rows cols floats).
 static int colsum(const float* X, int64_t rows, int64_t cols, int64_t ld, float* out,
-                  hipStream_t st) {
-  hipLaunchKernelGGL(colsum_kernel, dim3(ceil_div(cols, 128)), dim3(128), 0, st, X, rows, cols,
-                     ld, out);
+                  void* part, hipStream_t st) {
+  if (cols == 0) return DMIC_OK;
+  const int64_t nchunk = (rows + kColChunk - 1) / kColChunk;
+  DMIC_REQUIRE(nchunk <= 65535, DMIC_ERR_ARG, "too many rows for the column sum");
+  double* pd = static_cast<double*>(part);
+  if (nchunk == 1) {
+    hipLaunchKernelGGL(colsum_part_kernel, dim3(ceil_div(cols, 32), 1), dim3(256), 0, st, X, rows,
+                       cols, ld, pd, out);
+    DMIC_LAUNCH_CHECK();
+    return DMIC_OK;
+  }
+  if (nchunk > 1) {
+    hipLaunchKernelGGL(colsum_part_kernel, dim3(ceil_div(cols, 32), (unsigned)nchunk), dim3(256),
+                       0, st, X, rows, cols, ld, pd, (float*)nullptr);
+    DMIC_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(colsum_final_kernel, dim3(ceil_div(cols, 128)), dim3(128), 0, st, pd, nchunk,
+                     cols, out);
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
 }
@@ -211,28 +329,31 @@ int dmic_mlp_backward(const uint16_t* w_bf16, const float* x, const float* acts,
   const float* H3 = acts + 2 * rows * kHid;
   float* D1 = scratch;
   float* D2 = scratch + rows * kHid;
+  // the split-K partials of each weight gradient (ceil(rows/256) x 128 x 128
+  // floats, <= rows x 128 where split K is used) and the column-sum partials
+  // go to whichever half of scratch is free at that point
   // layer 4: dW4 = H3^T dy, db4 = colsum(dy), dH3 = (dy W4^T) * [H3 > 0]
-  DMIC_TRY((gemm<false, false>(H3, 1, kHid, dy, kOut, 1, grad_w + kW4, kOut, kHid, kOut, rows,
-                               nullptr, nullptr, 0, EPI_NONE, stream)));
-  DMIC_TRY(colsum(dy, rows, kOut, kOut, grad_b + kB4, stream));
+  DMIC_TRY((gemm_wgrad<false, false>(H3, 1, kHid, dy, kOut, 1, grad_w + kW4, kOut, kHid, kOut,
+                                     rows, D2, stream)));
+  DMIC_TRY(colsum(dy, rows, kOut, kOut, grad_b + kB4, D1, stream));
   DMIC_TRY((gemm<false, true>(dy, kOut, 1, w_bf16 + kW4, 1, kOut, D1, kHid, rows, kHid, kOut,
                               nullptr, H3, kHid, EPI_RELU_MASK, stream)));
   // layer 3
-  DMIC_TRY((gemm<false, false>(H2, 1, kHid, D1, kHid, 1, grad_w + kW3, kHid, kHid, kHid, rows,
-                               nullptr, nullptr, 0, EPI_NONE, stream)));
-  DMIC_TRY(colsum(D1, rows, kHid, kHid, grad_b + kB3, stream));
+  DMIC_TRY((gemm_wgrad<false, false>(H2, 1, kHid, D1, kHid, 1, grad_w + kW3, kHid, kHid, kHid,
+                                     rows, D2, stream)));
+  DMIC_TRY(colsum(D1, rows, kHid, kHid, grad_b + kB3, D2, stream));
   DMIC_TRY((gemm<false, true>(D1, kHid, 1, w_bf16 + kW3, 1, kHid, D2, kHid, rows, kHid, kHid,
                               nullptr, H2, kHid, EPI_RELU_MASK, stream)));
   // layer 2
-  DMIC_TRY((gemm<false, false>(H1, 1, kHid, D2, kHid, 1, grad_w + kW2, kHid, kHid, kHid, rows,
-                               nullptr, nullptr, 0, EPI_NONE, stream)));
-  DMIC_TRY(colsum(D2, rows, kHid, kHid, grad_b + kB2, stream));
+  DMIC_TRY((gemm_wgrad<false, false>(H1, 1, kHid, D2, kHid, 1, grad_w + kW2, kHid, kHid, kHid,
+                                     rows, D1, stream)));
+  DMIC_TRY(colsum(D2, rows, kHid, kHid, grad_b + kB2, D1, stream));
   DMIC_TRY((gemm<false, true>(D2, kHid, 1, w_bf16 + kW2, 1, kHid, D1, kHid, rows, kHid, kHid,
                               nullptr, H1, kHid, EPI_RELU_MASK, stream)));
   // layer 1
-  DMIC_TRY((gemm<false, false>(x, 1, kIn, D1, kHid, 1, grad_w + kW1, kHid, kIn, kHid, rows,
-                               nullptr, nullptr, 0, EPI_NONE, stream)));
-  DMIC_TRY(colsum(D1, rows, kHid, kHid, grad_b + kB1, stream));
+  DMIC_TRY((gemm_wgrad<false, false>(x, 1, kIn, D1, kHid, 1, grad_w + kW1, kHid, kIn, kHid, rows,
+                                     D2, stream)));
+  DMIC_TRY(colsum(D1, rows, kHid, kHid, grad_b + kB1, D2, stream));
   return DMIC_OK;
 }
 

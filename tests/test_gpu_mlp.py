@@ -84,17 +84,20 @@ def _torch_mlp(net, x):
     return h, Ws, bs
 
 
-def test_mlp_forward_backward_vs_torch(device):
+# rows: one column-sum chunk (<= 256 rows), several chunks, and split-K
+# weight gradients (>= 1024 rows, 256-row chunks, ragged last chunk)
+@pytest.mark.parametrize("rows", [1, 200, 1000, 5000])
+def test_mlp_forward_backward_vs_torch(device, rows):
     from deltamic_amd.proposal import ProposalMLP
     net = ProposalMLP(device, seed=3)
     net.b.normal_(0, 0.1)
     g = torch.Generator(device="cpu").manual_seed(9)
-    x = torch.randn(1000, N_IN, generator=g).to(device)
+    x = torch.randn(rows, N_IN, generator=g).to(device)
     y, acts = net.forward(x, keep_acts=True)
     ref, Ws, bs = _torch_mlp(net, x)
     y = y.cpu()
     assert torch.allclose(y, ref, rtol=1e-2, atol=1e-2 * ref.abs().max().item())
-    dy = torch.randn(1000, 3, generator=g).to(device)
+    dy = torch.randn(rows, 3, generator=g).to(device)
     gw, gb = net.backward(x, acts, dy)
     gw, gb, x, dy = gw.cpu(), gb.cpu(), x.cpu(), dy.cpu()
     # autograd of the reference wrt the (bf16-rounded) weights and biases
@@ -110,6 +113,11 @@ def test_mlp_forward_backward_vs_torch(device):
     ref_gb = torch.cat([bb.grad for bb in br])
     assert (gw - ref_gw).norm() / ref_gw.norm() < 2e-2
     assert (gb - ref_gb).norm() / ref_gb.norm() < 2e-2
+    # the output-layer bias gradient is the plain column sum of dy (FP64 sums)
+    assert torch.allclose(gb[-3:].double(), dy.double().sum(0), rtol=1e-6, atol=1e-6)
+    # and is reproducible call to call
+    gw2, gb2 = net.backward(x.to(device), acts, dy.to(device))
+    assert torch.equal(gw2.cpu(), gw) and torch.equal(gb2.cpu(), gb)
 
 
 def test_mlp_training_reduces_loss_and_is_deterministic(device):
