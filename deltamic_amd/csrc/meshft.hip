@@ -20,6 +20,7 @@
 //   floating-point atomics anywhere: results are bitwise reproducible.
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -47,26 +48,20 @@ struct __align__(16) FaceRec {
 };
 static_assert(sizeof(FaceRec) == 64, "FaceRec layout");
 
-// Per-frequency record of the adjoint tiles (48 bytes, 3 x 16B LDS reads).
+// Per-frequency record of the adjoint tiles (48 bytes).  Every pair reads the
+// first 32 bytes (two 16-byte LDS broadcasts); X0 and X1 are read only where
+// a band row starts (the vertex phase's xi_0 and xi_1 terms are constant
+// along a row).
 struct __align__(16) FreqRec {
-  double X[3];  // xi in FP64
-  float x[3];   // xi in FP32
-  float gr, gi; // upstream gradient g = dL/dRe + i dL/dIm
-  float inv2;   // autofluorescence variant: 1 / |xi|^2 (0 at xi = 0); else 0
+  double X2;      // xi_2 in FP64
+  float x[3];     // xi in FP32
+  float gr, gi;   // upstream gradient g = dL/dRe + i dL/dIm
+  float inv2;     // autofluorescence variant: 1 / |xi|^2 (0 at xi = 0); else 0
+  double X0, X1;  // xi_0, xi_1 in FP64
 };
 static_assert(sizeof(FreqRec) == 48, "FreqRec layout");
 
-constexpr int kTile = 256;        // faces (forward) / frequencies (adjoint) per LDS tile
-constexpr int kFold = 64;         // FP32 partial sums fold into FP64 every kFold pairs
-#ifndef DMIC_TARGET_BLOCKS
-#define DMIC_TARGET_BLOCKS 16384
-#endif
-// Workgroups a launch is split into (frequency blocks x face splits, face
-// blocks x frequency chunks).  The resident capacity is 256 CUs x 2-3
-// workgroups; 16384 makes the last partial round of workgroups a few percent
-// of the kernel (on the 1.08M-frequency band of a 20480-face sphere: 2048 -> 64.7
-// ms per forward+adjoint, 8192 -> 53.8 ms, 16384 -> 52.6 ms on MI355X).
-constexpr int kTargetBlocks = DMIC_TARGET_BLOCKS;
+constexpr int kFold = 64;  // FP32 partial sums fold into FP64 every kFold pairs
 
 // -------------------------------------------------------------------- prep
 
@@ -210,13 +205,22 @@ __device__ __forceinline__ float face_extent2(float b0, float b1, float b2, floa
   return __builtin_fmaxf(nb, __builtin_fmaxf(nc, nd));
 }
 
-// Maximum of v over the 256 threads of the workgroup, returned to every thread.
-__device__ __forceinline__ float block_max256(float v, float* red) {
+
+// Maximum of v over the 64 lanes of the wave, returned to every lane.
+__device__ __forceinline__ float wave_max64(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = __builtin_fmaxf(v, __shfl_xor(v, o));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  return __builtin_fmaxf(__builtin_fmaxf(red[0], red[1]), __builtin_fmaxf(red[2], red[3]));
+  return v;
+}
+
+// Orders this wave's LDS writes before its reads of other lanes' records (and
+// the reads of one tile before the writes of the next): LDS operations of a
+// wave complete in order, so only the compiler has to be kept from moving
+// them across.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // A pair of a tile can exceed kWidePhase only if (max |xi| of the block)^2
@@ -270,19 +274,61 @@ __device__ __forceinline__ void fwd_tile(const FaceRec* tile, int cnt, bool may_
   }
 }
 
+// Forward workgroup: kFwdWaves waves on the same 64 band frequencies (one per
+// lane), each wave on its own share of the workgroup's face piece with its
+// own LDS tile of 64 face records; the waves' FP64 sums are added in LDS in
+// a fixed order at the end.
+constexpr int kFwdWaves = 4;
+
+// Face piece of forward workgroup w: the frequency block (64 frequencies)
+// varies fastest.  levels == 0: the whole face range, written straight to
+// the output.  Else the face range is cut into `levels` geometric levels
+// (1/2, 1/4, ..., the last two equal), each into `parts` equal parts, large
+// pieces first, so the last workgroups of the launch are short; piece `slot`
+// writes partial slot `slot` ([nsplit][n] FP64 complex).
+struct FwdPiece {
+  int64_t f0, f1;
+  int slot;
+};
+__device__ __forceinline__ FwdPiece fwd_piece(int w, int64_t nf, int nfrb, int levels, int parts,
+                                              int& fr) {
+  FwdPiece c;
+  fr = w % nfrb;
+  c.slot = w / nfrb;
+  if (levels == 0) {
+    c.f0 = 0;
+    c.f1 = nf;
+    return c;
+  }
+  const int j = c.slot / parts, i = c.slot % parts;
+  const int64_t L0 = nf - (nf >> j);
+  const int64_t L1 = (j + 1 < levels) ? nf - (nf >> (j + 1)) : nf;
+  c.f0 = L0 + (L1 - L0) * i / parts;
+  c.f1 = L0 + (L1 - L0) * (i + 1) / parts;
+  return c;
+}
+
 // kAF: autofluorescence variant (fourier_from_mesh_narrowband_autofluorescence,
 // render.py:228-318): each face weighs w_t + i (xi . an_t) / |xi|^2, so the
 // lane also sums (xi . an_t) F_t and adds i/|xi|^2 times it at the end; at
 // xi = 0 that term is its limit -regint (the solid's transform at 0 over
-// regvol; the reference divides by zero there).
+// regvol; the reference divides by zero there), added by piece 0.
 template <bool kAF, typename T>
 __global__ __launch_bounds__(256) void meshft_fwd_kernel(
-    const FaceRec* __restrict__ faces, int64_t nf, int64_t faces_per_split,
-    const T* __restrict__ xi, int64_t n, T* __restrict__ out /* [n][2] */,
-    double2* __restrict__ part, const double* __restrict__ S_in, float regint) {
-  __shared__ FaceRec tile[kTile];
-  __shared__ float red[4];
-  const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi, int64_t n, int nfrb,
+    int levels, int parts, T* __restrict__ out /* [n][2] */, double2* __restrict__ part,
+    const double* __restrict__ S_in, float regint) {
+  constexpr int kAcc = kAF ? 4 : 2;  // FP64 sums per lane
+  constexpr size_t kTileBytes = sizeof(FaceRec) * kFwdWaves * 64;
+  constexpr size_t kRedBytes = sizeof(double) * (kFwdWaves - 1) * kAcc * 64;
+  __shared__ __align__(16) unsigned char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
+  // the wave index as a scalar (loop bounds and tile address stay scalar)
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = (int)threadIdx.x & 63;
+  FaceRec* tile = reinterpret_cast<FaceRec*>(smem) + wv * 64;
+  int fr;
+  const FwdPiece pc = fwd_piece((int)blockIdx.x, nf, nfrb, levels, parts, fr);
+  const int64_t m = (int64_t)fr * 64 + lane;
   const bool active = m < n;
   double X = 0.0, Y = 0.0, Z = 0.0;  // the band frequency, exact (float or double input)
   if (active) {
@@ -291,39 +337,57 @@ __global__ __launch_bounds__(256) void meshft_fwd_kernel(
     Z = xi[3 * m + 2];
   }
   const float x = (float)X, y = (float)Y, z = (float)Z;
-  const float x2max = block_max256(x * x + y * y + z * z, red);
-  const int64_t f0 = (int64_t)blockIdx.y * faces_per_split;
-  const int64_t f1 = (f0 + faces_per_split < nf) ? f0 + faces_per_split : nf;
-  double accr = 0.0, acci = 0.0, acnr = 0.0, acni = 0.0;
-  for (int64_t base = f0; base < f1; base += kTile) {
-    const int cnt = (int)((f1 - base < kTile) ? f1 - base : kTile);
-    __syncthreads();
-    int may_be_wide = 0;
-    if ((int)threadIdx.x < cnt) {
-      const FaceRec r = faces[base + threadIdx.x];
-      tile[threadIdx.x] = r;
-      may_be_wide = face_extent2(r.eb[0], r.eb[1], r.eb[2], r.ec[0], r.ec[1], r.ec[2]) * x2max >
-                    kWidePhase2;
+  const float x2max = wave_max64(x * x + y * y + z * z);
+  const int64_t g0 = pc.f0 + (pc.f1 - pc.f0) * wv / kFwdWaves;
+  const int64_t g1 = pc.f0 + (pc.f1 - pc.f0) * (wv + 1) / kFwdWaves;
+  double acc[kAcc];
+#pragma unroll
+  for (int i = 0; i < kAcc; ++i) acc[i] = 0.0;
+  for (int64_t base = g0; base < g1; base += 64) {
+    const int cnt = (int)((g1 - base < 64) ? g1 - base : 64);
+    bool wide = false;
+    if (lane < cnt) {
+      const FaceRec r = faces[base + lane];
+      tile[lane] = r;
+      wide = face_extent2(r.eb[0], r.eb[1], r.eb[2], r.ec[0], r.ec[1], r.ec[2]) * x2max >
+             kWidePhase2;
     }
-    const bool tile_may_be_wide = __syncthreads_or(may_be_wide) != 0;
-    fwd_tile<kAF>(tile, cnt, tile_may_be_wide, x, y, z, X, Y, Z, accr, acci, acnr, acni);
+    const bool may_be_wide = __builtin_amdgcn_ballot_w64(wide) != 0;
+    wave_lds_sync();
+    fwd_tile<kAF>(tile, cnt, may_be_wide, x, y, z, X, Y, Z, acc[0], acc[1], acc[kAcc - 2],
+                  acc[kAcc - 1]);
+    wave_lds_sync();
   }
-  if (kAF && active) {
+  // add the waves' sums in a fixed order (the tiles' LDS is reused)
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(smem);
+  if (wv > 0) {
+#pragma unroll
+    for (int i = 0; i < kAcc; ++i) red[((wv - 1) * kAcc + i) * 64 + lane] = acc[i];
+  }
+  __syncthreads();
+  if (wv != 0 || !active) return;
+#pragma unroll
+  for (int i = 0; i < kAcc; ++i) {
+#pragma unroll
+    for (int w = 0; w < kFwdWaves - 1; ++w) acc[i] += red[(w * kAcc + i) * 64 + lane];
+  }
+  double accr = acc[0], acci = acc[1];
+  if (kAF) {
+    const double acnr = acc[2], acni = acc[3];
     const double q = X * X + Y * Y + Z * Z;
     if (q > 0.0) {
       accr -= acni / q;
       acci += acnr / q;
-    } else if (blockIdx.y == 0 && S_in[1] != 0.0) {
+    } else if (pc.slot == 0 && S_in[1] != 0.0) {
       accr -= (double)regint;
     }
   }
-  if (active) {
-    if (part) {
-      part[(int64_t)blockIdx.y * n + m] = make_double2(accr, acci);
-    } else {
-      out[2 * m] = (T)accr;
-      out[2 * m + 1] = (T)acci;
-    }
+  if (part) {
+    part[(int64_t)pc.slot * n + m] = make_double2(accr, acci);
+  } else {
+    out[2 * m] = (T)accr;
+    out[2 * m + 1] = (T)acci;
   }
 }
 
@@ -353,6 +417,13 @@ __global__ __launch_bounds__(256) void fwd_split_reduce_kernel(const double2* __
 template <bool kAF>
 constexpr int kFaceSums = kAF ? 13 : 10;
 
+// Adjoint workgroup: kBwdWaves waves on the same 64 faces (one per lane),
+// each wave on its own share of the workgroup's frequency chunk with its
+// own LDS tile of 64 frequency records; the waves' sums are added in LDS in
+// a fixed order at the end, so one partial per face and chunk is written.
+constexpr int kBwdWaves = 4;
+constexpr int kBwdFaces = 64;
+
 // This lane's face in the adjoint (zeros for lanes past the last face).
 struct FaceLane {
   double A0, A1, A2;            // vertex a / (2 pi)
@@ -362,85 +433,133 @@ struct FaceLane {
 };
 
 // One LDS tile of the adjoint: this lane's face against cnt frequencies,
-// adding to the face's K sums (may_be_wide as in fwd_tile).
+// adding to the face's K sums.  Bit k of `cont` is set when record k lies on
+// the same band row as record k - 1 (same xi_0 and xi_1), so the vertex
+// phase's FP64 row term A0 X0 + A1 X1 is reused; bit 0 is never set.
+// may_be_wide: some pair of the tile may have a phase gap above kWidePhase
+// (then every pair takes the wave-wide check of wave_is_wide).
 template <bool kAF>
-__device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, bool may_be_wide,
-                                         const FaceLane& L, double (&acc)[kFaceSums<kAF>]) {
+__device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t cont,
+                                         bool may_be_wide, const FaceLane& L,
+                                         double (&acc)[kFaceSums<kAF>]) {
   constexpr int K = kFaceSums<kAF>;
-  for (int k0 = 0; k0 < cnt; k0 += kFold) {
-    const int k1 = (k0 + kFold < cnt) ? k0 + kFold : cnt;
-    float a[K];
+  float a[K];
 #pragma unroll
-    for (int i = 0; i < K; ++i) a[i] = 0.0f;
-    for (int k = k0; k < k1; ++k) {
-      const FreqRec& fr = tile[k];
-      const double rev_a = fma(L.A2, fr.X[2], fma(L.A1, fr.X[1], L.A0 * fr.X[0]));
-      const float p =
-          __builtin_fmaf(L.b2, fr.x[2], __builtin_fmaf(L.b1, fr.x[1], L.b0 * fr.x[0]));
-      const float q =
-          __builtin_fmaf(L.c2, fr.x[2], __builtin_fmaf(L.c1, fr.x[1], L.c0 * fr.x[0]));
-      const Sorted so = sort3(p, q);
-      cf em, d, pP, pQ;
-      float rD;
-      if (!may_be_wide || !wave_is_wide(so.D)) {
-        em = expmi_rev(mid_rev<false>(rev_a, so.mid));
-        face_H_grad<false>(so, d, pP, pQ, rD);
-      } else {
-        em = expmi_rev(mid_rev<true>(rev_a, so.mid));
-        face_H_grad<true>(so, d, pP, pQ, rD);
-      }
-      // Contract with conj(g) first, scaled by rD: z = conj(g) e_mid rD, then
-      // with H = d rD
-      //   rho = Re(conj g F) = Re(z d),  gamma = Re(conj g (-iF)) = Im(z d),
-      //   Re(conj g dF/dx_lo) = Re(z pP) + rho rD,
-      //   Re(conj g dF/dx_hi) = -(Re(z pQ) + rho rD),
-      //   Re(conj g dF/dx_mid) = gamma - (lo) - (hi).
-      const cf e = cmul({fr.gr, -fr.gi}, em);
-      const cf z = {e.re * rD, e.im * rD};
-      const float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im);
-      cf zt = z;      // conj(g) e_mid rD times the face weight when it sits inside the sum
-      float rt = rh;  // Re(zt d)
-      if (kAF) {
-        // z2 = i conj(g) e_mid rD / |xi|^2; zt = w z + s z2 with s = xi . an
-        const cf z2 = {-z.im * fr.inv2, z.re * fr.inv2};
-        const float sn = __builtin_fmaf(L.n2, fr.x[2], __builtin_fmaf(L.n1, fr.x[1], L.n0 * fr.x[0]));
-        zt = {__builtin_fmaf(L.fw, z.re, sn * z2.re), __builtin_fmaf(L.fw, z.im, sn * z2.im)};
-        rt = __builtin_fmaf(zt.re, d.re, -zt.im * d.im);
-        const float nn = __builtin_fmaf(z2.re, d.re, -z2.im * d.im);
-#pragma unroll
-        for (int j = 0; j < 3; ++j) a[10 + j] = __builtin_fmaf(nn, fr.x[j], a[10 + j]);
-      }
-      const float rtD = rt * rD;
-      const float ga = __builtin_fmaf(zt.re, d.im, zt.im * d.re);
-      const float alo = __builtin_fmaf(zt.re, pP.re, __builtin_fmaf(-zt.im, pP.im, rtD));
-      const float ahi = -__builtin_fmaf(zt.re, pQ.re, __builtin_fmaf(-zt.im, pQ.im, rtD));
-      const float amid = ga - alo - ahi;
-      const int ra = (p < 0.0f) + (q < 0.0f);
-      const int rb = (p >= 0.0f) + (q < p);
-      const float al = (ra == 0) ? alo : ((ra == 2) ? ahi : amid);
-      const float be = (rb == 0) ? alo : ((rb == 2) ? ahi : amid);
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        a[j] = __builtin_fmaf(al, fr.x[j], a[j]);
-        a[3 + j] = __builtin_fmaf(be, fr.x[j], a[3 + j]);
-        a[6 + j] = __builtin_fmaf(ga, fr.x[j], a[6 + j]);
-      }
-      a[9] += rh;
+  for (int i = 0; i < K; ++i) a[i] = 0.0f;
+  double row = 0.0;
+  for (int k = 0; k < cnt; ++k) {
+    const FreqRec& fr = tile[k];
+    if (!((cont >> k) & 1)) row = fma(L.A1, fr.X1, L.A0 * fr.X0);  // wave-uniform
+    const double rev_a = fma(L.A2, fr.X2, row);
+    const float p =
+        __builtin_fmaf(L.b2, fr.x[2], __builtin_fmaf(L.b1, fr.x[1], L.b0 * fr.x[0]));
+    const float q =
+        __builtin_fmaf(L.c2, fr.x[2], __builtin_fmaf(L.c1, fr.x[1], L.c0 * fr.x[0]));
+    const Sorted so = sort3(p, q);
+    cf em, d, pP, pQ;
+    float rD;
+    if (!may_be_wide || !wave_is_wide(so.D)) {
+      em = expmi_rev(mid_rev<false>(rev_a, so.mid));
+      face_H_grad<false>(so, d, pP, pQ, rD);
+    } else {
+      em = expmi_rev(mid_rev<true>(rev_a, so.mid));
+      face_H_grad<true>(so, d, pP, pQ, rD);
     }
+    // Contract with conj(g) first, scaled by rD: z = conj(g) e_mid rD, then
+    // with H = d rD
+    //   rho = Re(conj g F) = Re(z d),  gamma = Re(conj g (-iF)) = Im(z d),
+    //   Re(conj g dF/dx_lo) = Re(z pP) + rho rD,
+    //   Re(conj g dF/dx_hi) = -(Re(z pQ) + rho rD),
+    //   Re(conj g dF/dx_mid) = gamma - (lo) - (hi).
+    const cf e = cmul({fr.gr, -fr.gi}, em);
+    const cf z = {e.re * rD, e.im * rD};
+    const float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im);
+    cf zt = z;      // conj(g) e_mid rD times the face weight when it sits inside the sum
+    float rt = rh;  // Re(zt d)
+    if (kAF) {
+      // z2 = i conj(g) e_mid rD / |xi|^2; zt = w z + s z2 with s = xi . an
+      const cf z2 = {-z.im * fr.inv2, z.re * fr.inv2};
+      const float sn = __builtin_fmaf(L.n2, fr.x[2], __builtin_fmaf(L.n1, fr.x[1], L.n0 * fr.x[0]));
+      zt = {__builtin_fmaf(L.fw, z.re, sn * z2.re), __builtin_fmaf(L.fw, z.im, sn * z2.im)};
+      rt = __builtin_fmaf(zt.re, d.re, -zt.im * d.im);
+      const float nn = __builtin_fmaf(z2.re, d.re, -z2.im * d.im);
 #pragma unroll
-    for (int i = 0; i < K; ++i) acc[i] += a[i];
+      for (int j = 0; j < 3; ++j) a[10 + j] = __builtin_fmaf(nn, fr.x[j], a[10 + j]);
+    }
+    const float rtD = rt * rD;
+    const float ga = __builtin_fmaf(zt.re, d.im, zt.im * d.re);
+    const float alo = __builtin_fmaf(zt.re, pP.re, __builtin_fmaf(-zt.im, pP.im, rtD));
+    const float ahi = -__builtin_fmaf(zt.re, pQ.re, __builtin_fmaf(-zt.im, pQ.im, rtD));
+    const float amid = ga - alo - ahi;
+    const int ra = (p < 0.0f) + (q < 0.0f);
+    const int rb = (p >= 0.0f) + (q < p);
+    const float al = (ra == 0) ? alo : ((ra == 2) ? ahi : amid);
+    const float be = (rb == 0) ? alo : ((rb == 2) ? ahi : amid);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      a[j] = __builtin_fmaf(al, fr.x[j], a[j]);
+      a[3 + j] = __builtin_fmaf(be, fr.x[j], a[3 + j]);
+      a[6 + j] = __builtin_fmaf(ga, fr.x[j], a[6 + j]);
+    }
+    a[9] += rh;
   }
+#pragma unroll
+  for (int i = 0; i < K; ++i) acc[i] += a[i];
+}
+
+// Work decomposition of the adjoint (BwdPlan): workgroup w takes face block
+// fb (64 faces) and one frequency chunk.  Workgroups are dispatched
+// round-robin over the 8 XCDs, so with `levels` > 0 the workgroups with
+// w % 8 == x take the x-th eighth of the band, which then streams through
+// that XCD's L2 once.  Each eighth is cut into `levels` geometric levels
+// (1/2, 1/4, ..., the last two equal) and each level into `parts` equal
+// parts; the face block varies fastest and the large pieces come first, so
+// the last workgroups of the launch are short and the launch ends with
+// little idle time.  levels == 0: nchunk equal chunks, no XCD grouping.
+struct BwdChunk {
+  int64_t m0, m1;  // frequency range
+  int chunk;       // partial slot
+};
+__device__ __forceinline__ BwdChunk bwd_chunk(int w, int64_t n, int nfb, int nchunk, int levels,
+                                              int parts, int& fb) {
+  BwdChunk c;
+  if (levels == 0) {
+    fb = w % nfb;
+    c.chunk = w / nfb;
+    c.m0 = n * c.chunk / nchunk;
+    c.m1 = n * (c.chunk + 1) / nchunk;
+    return c;
+  }
+  const int x = w & 7, s = w >> 3;
+  fb = s % nfb;
+  const int idx = s / nfb, j = idx / parts, i = idx % parts;
+  const int64_t X0 = n * x / 8, len = n * (x + 1) / 8 - X0;
+  const int64_t L0 = X0 + len - (len >> j);
+  const int64_t L1 = (j + 1 < levels) ? X0 + len - (len >> (j + 1)) : X0 + len;
+  c.m0 = L0 + (L1 - L0) * i / parts;
+  c.m1 = L0 + (L1 - L0) * (i + 1) / parts;
+  c.chunk = x * levels * parts + idx;
+  return c;
 }
 
 template <bool kAF, typename T>
 __global__ __launch_bounds__(256) void meshft_bwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi,
-    const T* __restrict__ grad, int64_t n, int64_t freqs_per_chunk,
+    const T* __restrict__ grad, int64_t n, int nfb, int nchunk, int levels, int parts,
     double* __restrict__ part /* [nchunk][kFaceSums][nf] */) {
   constexpr int K = kFaceSums<kAF>;
-  __shared__ FreqRec tile[kTile];
-  __shared__ float red[4];
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  constexpr size_t kTileBytes = sizeof(FreqRec) * kBwdWaves * 64;
+  constexpr size_t kRedBytes = sizeof(double) * (kBwdWaves - 1) * K * kBwdFaces;
+  __shared__ __align__(16) unsigned char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
+  // the wave index as a scalar: the compiler then keeps the frequency loop's
+  // bounds and the tile address in scalar registers
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = (int)threadIdx.x & 63;
+  FreqRec* tile = reinterpret_cast<FreqRec*>(smem) + wv * 64;
+  int fb;
+  const BwdChunk ck = bwd_chunk((int)blockIdx.x, n, nfb, nchunk, levels, parts, fb);
+  const int chunk = ck.chunk;
+  const int64_t t = (int64_t)fb * kBwdFaces + lane;
   const bool active = t < nf;
   FaceLane L = {};
   if (active) {
@@ -451,35 +570,58 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
     L.fw = -0.5f * r.w;  // the face weight 2 c_t A_t / sum c A (exact)
     L.n0 = -0.5f * r.an[0]; L.n1 = -0.5f * r.an[1]; L.n2 = -0.5f * r.an[2];  // exact
   }
-  const float e2max = block_max256(face_extent2(L.b0, L.b1, L.b2, L.c0, L.c1, L.c2), red);
-  const int64_t m0 = (int64_t)blockIdx.y * freqs_per_chunk;
-  const int64_t m1 = (m0 + freqs_per_chunk < n) ? m0 + freqs_per_chunk : n;
+  const float e2max = wave_max64(face_extent2(L.b0, L.b1, L.b2, L.c0, L.c1, L.c2));
+  // this wave's share of the chunk
+  const int64_t q0 = ck.m0 + (ck.m1 - ck.m0) * wv / kBwdWaves;
+  const int64_t q1 = ck.m0 + (ck.m1 - ck.m0) * (wv + 1) / kBwdWaves;
   double acc[K];
 #pragma unroll
   for (int i = 0; i < K; ++i) acc[i] = 0.0;
-  for (int64_t base = m0; base < m1; base += kTile) {
-    const int cnt = (int)((m1 - base < kTile) ? m1 - base : kTile);
-    __syncthreads();
-    int may_be_wide = 0;
-    if ((int)threadIdx.x < cnt) {
-      const int64_t m = base + threadIdx.x;
+  for (int64_t base = q0; base < q1; base += 64) {
+    const int cnt = (int)((q1 - base < 64) ? q1 - base : 64);
+    bool wide = false, same_row = false;
+    if (lane < cnt) {
+      const int64_t m = base + lane;
+      const T* xm = xi + 3 * m;
       FreqRec fr;
-      fr.X[0] = xi[3 * m]; fr.X[1] = xi[3 * m + 1]; fr.X[2] = xi[3 * m + 2];
-      fr.x[0] = (float)fr.X[0]; fr.x[1] = (float)fr.X[1]; fr.x[2] = (float)fr.X[2];
+      fr.X0 = xm[0]; fr.X1 = xm[1]; fr.X2 = xm[2];
+      fr.x[0] = (float)fr.X0; fr.x[1] = (float)fr.X1; fr.x[2] = (float)fr.X2;
       fr.gr = (float)grad[2 * m];
       fr.gi = (float)grad[2 * m + 1];
-      const double q = fr.X[0] * fr.X[0] + fr.X[1] * fr.X[1] + fr.X[2] * fr.X[2];
-      fr.inv2 = (kAF && q > 0.0) ? (float)(1.0 / q) : 0.0f;
-      tile[threadIdx.x] = fr;
-      may_be_wide = (float)q * e2max > kWidePhase2;
+      if (kAF) {
+        const double q = fr.X0 * fr.X0 + fr.X1 * fr.X1 + fr.X2 * fr.X2;
+        fr.inv2 = (q > 0.0) ? (float)(1.0 / q) : 0.0f;
+      } else {
+        fr.inv2 = 0.0f;
+      }
+      tile[lane] = fr;
+      const float q32 = fr.x[0] * fr.x[0] + fr.x[1] * fr.x[1] + fr.x[2] * fr.x[2];
+      wide = q32 * e2max > kWidePhase2;
+      same_row = lane > 0 && xm[-3] == xm[0] && xm[-2] == xm[1];
     }
-    const bool tile_may_be_wide = __syncthreads_or(may_be_wide) != 0;
-    bwd_tile<kAF>(tile, cnt, tile_may_be_wide, L, acc);
+    const uint64_t cont = __builtin_amdgcn_ballot_w64(same_row);
+    const bool may_be_wide = __builtin_amdgcn_ballot_w64(wide) != 0;
+    wave_lds_sync();
+    bwd_tile<kAF>(tile, cnt, cont, may_be_wide, L, acc);
+    wave_lds_sync();
   }
-  if (active) {
-    double* dst = part + (int64_t)blockIdx.y * K * nf + t;
+  // add the waves' sums in a fixed order (the tiles' LDS is reused)
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(smem);
+  if (wv > 0) {
 #pragma unroll
-    for (int i = 0; i < K; ++i) dst[(int64_t)i * nf] = acc[i];
+    for (int i = 0; i < K; ++i) red[((wv - 1) * K + i) * kBwdFaces + lane] = acc[i];
+  }
+  __syncthreads();
+  if (wv == 0 && active) {
+    double* dst = part + (int64_t)chunk * K * nf + t;
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      double s = acc[i];
+#pragma unroll
+      for (int w = 0; w < kBwdWaves - 1; ++w) s += red[(w * K + i) * kBwdFaces + lane];
+      dst[(int64_t)i * nf] = s;
+    }
   }
 }
 
@@ -729,46 +871,119 @@ static int timer_end(int k, hipStream_t st) {
 }
 
 
+// Workgroups of `kernel` (256 threads) resident on the whole current device:
+// CUs x workgroups per CU from the runtime's occupancy calculator, cached per
+// device and kernel.  Without a device (workspace queries on a host with no
+// GPU) the MI355X figure, 256 CUs x fallback_per_cu, is used.
+static int resident_wgs(const void* kernel, int fallback_per_cu) {
+  constexpr int kSlots = 16;
+  static std::mutex mu;
+  static int dev_of[kSlots], val_of[kSlots];
+  static const void* ker_of[kSlots];
+  static int used = 0;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    (void)hipGetLastError();
+    return 256 * fallback_per_cu;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  for (int i = 0; i < used; ++i)
+    if (dev_of[i] == dev && ker_of[i] == kernel) return val_of[i];
+  int cus = 0, per = 0;
+  int v = 256 * fallback_per_cu;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) == hipSuccess &&
+      cus > 0 && per > 0)
+    v = cus * per;
+  else
+    (void)hipGetLastError();
+  if (used < kSlots) {
+    dev_of[used] = dev;
+    ker_of[used] = kernel;
+    val_of[used] = v;
+    ++used;
+  }
+  return v;
+}
+
+// Geometric face levels of the forward (DMIC_FWD_LEVELS overrides).
+static int fwd_levels() {
+  static const int r = [] {
+    const char* e = getenv("DMIC_FWD_LEVELS");
+    const int v = e ? atoi(e) : -1;
+    return (v >= 0 && v <= 20) ? v : 4;
+  }();
+  return r;
+}
+
 struct FwdPlan {
-  int64_t faces_per_split;
-  int nsplit;
+  int nfrb;    // frequency blocks of 64
+  int levels;  // 0: one piece per frequency block, no partials
+  int parts;
+  int nsplit;  // pieces per frequency block (partial slots when > 1)
 };
 
-// part_cap (>= 0) bounds the split partials to part_cap double2 entries
-// (nsplit * n); a band too large for two splits is evaluated unsplit.
+// part_cap (>= 0) bounds the partials to part_cap double2 entries (nsplit * n):
+// fewer levels, or one piece, when they do not fit.
 static FwdPlan fwd_plan(int64_t nf, int64_t n, int64_t part_cap = -1) {
-  FwdPlan p{nf > 0 ? nf : 1, 1};
-  const int64_t fblocks = (n + 255) / 256;
-  if (fblocks <= 0 || nf <= 0) return p;
-  int64_t want = (kTargetBlocks + fblocks - 1) / fblocks;
-  const int64_t maxsplit = (nf + kFold - 1) / kFold;
-  if (want > maxsplit) want = maxsplit;
-  if (part_cap >= 0 && want > part_cap / n) want = part_cap / n;
-  if (want < 2) want = 1;
-  int64_t per = (nf + want - 1) / want;
-  per = (per + kFold - 1) / kFold * kFold;
-  p.faces_per_split = per;
-  p.nsplit = (int)((nf + per - 1) / per);
+  FwdPlan p{(int)ceil_div(n > 0 ? n : 1, 64), 0, 1, 1};
+  if (nf <= 0 || n <= 0) return p;
+  const int cap = resident_wgs((const void*)meshft_fwd_kernel<false, float>, 8);
+  if ((int64_t)p.nfrb >= 32 * (int64_t)cap) return p;  // the band alone fills 32 rounds
+  constexpr int64_t kMinPiece = 64;                     // faces per piece (16 per wave)
+  const int64_t parts = (cap + (int64_t)p.nfrb - 1) / p.nfrb;
+  int levels = fwd_levels();
+  while (levels > 0 && (nf >> (levels > 1 ? levels - 1 : 0)) / parts < kMinPiece) --levels;
+  while (levels > 0 && part_cap >= 0 && (int64_t)levels * parts * n > part_cap) --levels;
+  if (levels == 0 || levels * parts < 2) return p;
+  p.levels = levels;
+  p.parts = (int)parts;
+  p.nsplit = levels * (int)parts;
   return p;
 }
 
+// Geometric levels per XCD eighth of the band (DMIC_BWD_LEVELS overrides).
+static int bwd_levels() {
+  static const int r = [] {
+    const char* e = getenv("DMIC_BWD_LEVELS");
+    const int v = e ? atoi(e) : -1;
+    return (v >= 0 && v <= 20) ? v : 6;
+  }();
+  return r;
+}
+
 struct BwdPlan {
-  int64_t freqs_per_chunk;
-  int nchunk;
+  int nfb;     // face blocks of kBwdFaces
+  int nchunk;  // frequency chunks (partial slots)
+  int levels;  // 0: nchunk equal chunks; else geometric levels per XCD eighth
+  int parts;   // equal parts per level
 };
 
+// The float kernel's occupancy sizes the plan for both boundary types, so the
+// float and double entry points share one workspace size.
+template <bool kAF>
 static BwdPlan bwd_plan(int64_t nf, int64_t n) {
-  BwdPlan p{n > 0 ? n : 1, 1};
-  const int64_t tblocks = (nf + 255) / 256;
-  if (tblocks <= 0 || n <= 0) return p;
-  int64_t want = (kTargetBlocks + tblocks - 1) / tblocks;
-  const int64_t maxc = (n + kFold - 1) / kFold;
-  if (want > maxc) want = maxc;
-  if (want < 1) want = 1;
-  int64_t per = (n + want - 1) / want;
-  per = (per + kFold - 1) / kFold * kFold;
-  p.freqs_per_chunk = per;
-  p.nchunk = (int)((n + per - 1) / per);
+  BwdPlan p{(int)ceil_div(nf > 0 ? nf : 1, kBwdFaces), 1, 0, 1};
+  if (nf <= 0 || n <= 0) return p;
+  const int cap = resident_wgs((const void*)meshft_bwd_kernel<kAF, float>, 5);
+  constexpr int64_t kMinPiece = kBwdWaves * 64;  // one full tile per wave
+  // parts per level: enough workgroups per level to fill the device
+  const int64_t parts = (cap + 8 * (int64_t)p.nfb - 1) / (8 * (int64_t)p.nfb);
+  const int64_t len = n / 8;
+  int levels = bwd_levels();
+  while (levels > 0 && (len >> (levels > 1 ? levels - 1 : 0)) / parts < kMinPiece) --levels;
+  if (levels > 0) {
+    p.levels = levels;
+    p.parts = (int)parts;
+    p.nchunk = 8 * levels * (int)parts;
+    return p;
+  }
+  // small bands: equal chunks, about 8 rounds of resident workgroups
+  int64_t c = (8 * (int64_t)cap + p.nfb - 1) / p.nfb;
+  const int64_t cmax = n / kMinPiece > 1 ? n / kMinPiece : 1;
+  if (c > cmax) c = cmax;
+  if (c < 1) c = 1;
+  p.nchunk = (int)c;
   return p;
 }
 
@@ -813,7 +1028,9 @@ struct BwdWs {
   int* ent;
 };
 
-static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n, bool af) {
+template <bool kAF>
+static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n) {
+  constexpr bool af = kAF;
   BwdWs w;
   const int64_t f1 = nf > 0 ? nf : 1, v1 = nv > 0 ? nv : 1;
   w.flag = c.take<int>(4);
@@ -821,8 +1038,8 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n, bool af) {
   w.S = c.take<double>(2);
   w.spart = c.take<double>(2 * (size_t)ceil_div(f1, 256));
   w.recs = c.take<FaceRec>(f1);
-  const BwdPlan p = bwd_plan(nf, n);
-  w.part = c.take<double>((size_t)p.nchunk * (af ? kFaceSums<true> : kFaceSums<false>) * f1);
+  const BwdPlan p = bwd_plan<kAF>(nf, n);
+  w.part = c.take<double>((size_t)p.nchunk * kFaceSums<kAF> * f1);
   w.G = c.take<double>(9 * f1);
   w.cgA = c.take<double>(9 * f1);
   w.wR = c.take<double>(f1);
@@ -876,11 +1093,12 @@ static int forward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, cons
                                w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
   const FwdPlan p = fwd_plan(nf, n, part_cap);
-  dim3 grid(ceil_div(n, 256), p.nsplit);
+  const dim3 grid((unsigned)((int64_t)p.nfrb * p.nsplit));
   rc = timer_begin(0, st);
   if (rc != DMIC_OK) return rc;
-  hipLaunchKernelGGL((meshft_fwd_kernel<kAF, T>), grid, dim3(256), 0, st, w.recs, nf,
-                     p.faces_per_split, xi, n, out, p.nsplit > 1 ? w.part : nullptr, w.S, regint);
+  hipLaunchKernelGGL((meshft_fwd_kernel<kAF, T>), grid, dim3(256), 0, st, w.recs, nf, xi, n,
+                     p.nfrb, p.levels, p.parts, out, p.nsplit > 1 ? w.part : nullptr, w.S,
+                     regint);
   DMIC_LAUNCH_CHECK();
   rc = timer_end(0, st);
   if (rc != DMIC_OK) return rc;
@@ -898,7 +1116,7 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
                          size_t ws_bytes, hipStream_t st) {
   constexpr int K = kFaceSums<kAF>;
   Carver c(ws);
-  BwdWs w = carve_bwd(c, nv, nf, n, kAF);
+  BwdWs w = carve_bwd<kAF>(c, nv, nf, n);
   DMIC_REQUIRE(ws_bytes >= c.bytes(), DMIC_ERR_WORKSPACE,
                "backward workspace too small: %zu < %zu bytes", ws_bytes, c.bytes());
   hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, w.flag, 4, 0);
@@ -914,13 +1132,13 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
   int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S,
                                w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
-  const BwdPlan p = bwd_plan(nf, n);
+  const BwdPlan p = bwd_plan<kAF>(nf, n);
   if (n > 0) {
-    dim3 grid(ceil_div(nf, 256), p.nchunk);
+    const dim3 grid((unsigned)((int64_t)p.nfb * p.nchunk));
     rc = timer_begin(1, st);
     if (rc != DMIC_OK) return rc;
     hipLaunchKernelGGL((meshft_bwd_kernel<kAF, T>), grid, dim3(256), 0, st, w.recs, nf, xi, grad,
-                       n, p.freqs_per_chunk, w.part);
+                       n, p.nfb, p.nchunk, p.levels, p.parts, w.part);
     DMIC_LAUNCH_CHECK();
     rc = timer_end(1, st);
     if (rc != DMIC_OK) return rc;
@@ -983,14 +1201,14 @@ size_t dmic_meshft_forward_workspace(int64_t nv, int64_t nf, int64_t n) {
 size_t dmic_meshft_backward_workspace(int64_t nv, int64_t nf, int64_t n) {
   if (nv < 0 || nf < 0 || n < 0) return 0;
   Carver c(nullptr);
-  carve_bwd(c, nv, nf, n, false);
+  carve_bwd<false>(c, nv, nf, n);
   return c.bytes();
 }
 
 size_t dmic_meshft_af_backward_workspace(int64_t nv, int64_t nf, int64_t n) {
   if (nv < 0 || nf < 0 || n < 0) return 0;
   Carver c(nullptr);
-  carve_bwd(c, nv, nf, n, true);
+  carve_bwd<true>(c, nv, nf, n);
   return c.bytes();
 }
 
