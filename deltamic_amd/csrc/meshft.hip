@@ -157,6 +157,17 @@ __global__ __launch_bounds__(1024) void face_sum_final_kernel(const double* __re
   }
 }
 
+// Squared bound on a face's phase gap per unit |xi|:
+// D = max(|xi.eb|, |xi.ec|, |xi.(ec - eb)|) <= |xi| max(|eb|, |ec|, |ec - eb|).
+__device__ __forceinline__ float face_extent2(float b0, float b1, float b2, float c0, float c1,
+                                              float c2) {
+  const float d0 = c0 - b0, d1 = c1 - b1, d2 = c2 - b2;
+  const float nb = b0 * b0 + b1 * b1 + b2 * b2;
+  const float nc = c0 * c0 + c1 * c1 + c2 * c2;
+  const float nd = d0 * d0 + d1 * d1 + d2 * d2;
+  return __builtin_fmaxf(nb, __builtin_fmaxf(nc, nd));
+}
+
 template <typename IdxT, typename T>
 __global__ __launch_bounds__(256) void face_pack_kernel(
     const T* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
@@ -190,21 +201,13 @@ __global__ __launch_bounds__(256) void face_pack_kernel(
   r.an[1] = (float)(sc * (u[2] * q[0] - u[0] * q[2]));
   r.an[2] = (float)(sc * (u[0] * q[1] - u[1] * q[0]));
   out[t] = r;
+  // largest squared face extent of the mesh (flag[1], zeroed with the index
+  // flag; non-negative floats order like their bit patterns)
+  atomicMax(flag + 1, __float_as_int(face_extent2(r.eb[0], r.eb[1], r.eb[2], r.ec[0], r.ec[1],
+                                                  r.ec[2])));
 }
 
 // -------------------------------------------------------------------- forward
-
-// Squared bound on a face's phase gap per unit |xi|:
-// D = max(|xi.eb|, |xi.ec|, |xi.(ec - eb)|) <= |xi| max(|eb|, |ec|, |ec - eb|).
-__device__ __forceinline__ float face_extent2(float b0, float b1, float b2, float c0, float c1,
-                                              float c2) {
-  const float d0 = c0 - b0, d1 = c1 - b1, d2 = c2 - b2;
-  const float nb = b0 * b0 + b1 * b1 + b2 * b2;
-  const float nc = c0 * c0 + c1 * c1 + c2 * c2;
-  const float nd = d0 * d0 + d1 * d1 + d2 * d2;
-  return __builtin_fmaxf(nb, __builtin_fmaxf(nc, nd));
-}
-
 
 // Maximum of v over the 64 lanes of the wave, returned to every lane.
 __device__ __forceinline__ float wave_max64(float v) {
@@ -223,188 +226,175 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// A pair of a tile can exceed kWidePhase only if (max |xi| of the block)^2
-// times (max face extent of the tile)^2 does.  Tiles where it cannot skip
-// the per-pair wave check (may_be_wide = false, a workgroup-uniform branch);
-// the others check every pair (wave_is_wide).
+// A pair can exceed kWidePhase only if |xi|^2 times the face's squared extent
+// does.  Where (max |xi| of the wave's frequencies)^2 times (max extent over
+// the mesh, from face_pack_kernel)^2 cannot, the per-pair wave check is
+// skipped (may_be_wide = false, a wave-uniform branch); elsewhere every pair
+// takes it (wave_is_wide).
 constexpr float kWidePhase2 = kWidePhase * kWidePhase;
 
-// One LDS tile of the forward: this lane's frequency against cnt faces.
+// Forward: one wave per workgroup, on kFwdFreqs<kAF> consecutive band frequencies
+// (their records in LDS, read as broadcasts) and every face, one face per
+// lane: face block b is faces [64 b, 64 b + 64).  Each lane sums w_t F_t of
+// its faces per frequency in FP32 registers and folds the sums into its own
+// FP64 slots in LDS every kFwdFold face blocks; at the end the 64 lanes'
+// FP64 sums are added in a fixed order.  Consecutive band entries mostly lie
+// on one band row (equal xi_0, xi_1), so the row terms of the vertex phase
+// (FP64) and the edge phases are formed once per row and face.
+// 8 frequencies per wave; 4 for the autofluorescence variant, whose four sums
+// per frequency would otherwise double the LDS and registers per wave
 template <bool kAF>
-__device__ __forceinline__ void fwd_tile(const FaceRec* tile, int cnt, bool may_be_wide, float x,
-                                         float y, float z, double X, double Y, double Z,
-                                         double& accr, double& acci, double& acnr, double& acni) {
-  for (int k0 = 0; k0 < cnt; k0 += kFold) {
-    const int k1 = (k0 + kFold < cnt) ? k0 + kFold : cnt;
-    float sr = 0.0f, si = 0.0f, tr = 0.0f, ti = 0.0f;
-    auto pair = [&](const FaceRec& r) {
-      const double rev_a = fma(r.a[2], Z, fma(r.a[1], Y, r.a[0] * X));
-      const float p = __builtin_fmaf(r.eb[2], z, __builtin_fmaf(r.eb[1], y, r.eb[0] * x));
-      const float q = __builtin_fmaf(r.ec[2], z, __builtin_fmaf(r.ec[1], y, r.ec[0] * x));
-      const Sorted so = sort3(p, q);
-      cf F;   // F = e_mid N, face value = -2 m F (r.w carries the -2)
-      float m;
-      if (!may_be_wide || !wave_is_wide(so.D))
-        F = cmul(expmi_rev(mid_rev<false>(rev_a, so.mid)), face_H<false>(so, m));
-      else
-        F = cmul(expmi_rev(mid_rev<true>(rev_a, so.mid)), face_H<true>(so, m));
-      const float wm = r.w * m;
-      sr = __builtin_fmaf(wm, F.re, sr);
-      si = __builtin_fmaf(wm, F.im, si);
-      if (kAF) {
-        const float tm = __builtin_fmaf(r.an[2], z, __builtin_fmaf(r.an[1], y, r.an[0] * x)) * m;
-        tr = __builtin_fmaf(tm, F.re, tr);
-        ti = __builtin_fmaf(tm, F.im, ti);
-      }
-    };
-    // two faces per iteration: one LDS base address serves both records
-    // (the loop holds a wave ballot, so the compiler will not unroll it)
-    int k = k0;
-    for (; k + 2 <= k1; k += 2) {
-      pair(tile[k]);
-      pair(tile[k + 1]);
-    }
-    if (k < k1) pair(tile[k]);
-    accr += (double)sr;
-    acci += (double)si;
-    if (kAF) {
-      acnr += (double)tr;
-      acni += (double)ti;
-    }
-  }
-}
+constexpr int kFwdFreqs = kAF ? 4 : 8;
+constexpr int kFwdFold = 64;
 
-// Forward workgroup: kFwdWaves waves on the same 64 band frequencies (one per
-// lane), each wave on its own share of the workgroup's face piece with its
-// own LDS tile of 64 face records; the waves' FP64 sums are added in LDS in
-// a fixed order at the end.
-constexpr int kFwdWaves = 4;
-
-// Face piece of forward workgroup w: the frequency block (64 frequencies)
-// varies fastest.  levels == 0: the whole face range, written straight to
-// the output.  Else the face range is cut into `levels` geometric levels
-// (1/2, 1/4, ..., the last two equal), each into `parts` equal parts, large
-// pieces first, so the last workgroups of the launch are short; piece `slot`
-// writes partial slot `slot` ([nsplit][n] FP64 complex).
-struct FwdPiece {
-  int64_t f0, f1;
-  int slot;
+template <bool kAF>
+struct FwdLds {
+  static constexpr int kV = kAF ? 4 : 2;  // sums per frequency: F (re, im) [+ (xi . an) F]
+  FreqRec rec[kFwdFreqs<kAF>];
+  double acc[kV * kFwdFreqs<kAF>][64];  // per-lane FP64 sums; reused for the lane reduction
 };
-__device__ __forceinline__ FwdPiece fwd_piece(int w, int64_t nf, int nfrb, int levels, int parts,
-                                              int& fr) {
-  FwdPiece c;
-  fr = w % nfrb;
-  c.slot = w / nfrb;
-  if (levels == 0) {
-    c.f0 = 0;
-    c.f1 = nf;
-    return c;
-  }
-  const int j = c.slot / parts, i = c.slot % parts;
-  const int64_t L0 = nf - (nf >> j);
-  const int64_t L1 = (j + 1 < levels) ? nf - (nf >> (j + 1)) : nf;
-  c.f0 = L0 + (L1 - L0) * i / parts;
-  c.f1 = L0 + (L1 - L0) * (i + 1) / parts;
-  return c;
-}
 
 // kAF: autofluorescence variant (fourier_from_mesh_narrowband_autofluorescence,
 // render.py:228-318): each face weighs w_t + i (xi . an_t) / |xi|^2, so the
-// lane also sums (xi . an_t) F_t and adds i/|xi|^2 times it at the end; at
-// xi = 0 that term is its limit -regint (the solid's transform at 0 over
-// regvol; the reference divides by zero there), added by piece 0.
+// lane also sums (xi . an_t) F_t and i/|xi|^2 times it is added at the end;
+// at xi = 0 that term is its limit -regint (the solid's transform at 0 over
+// regvol; the reference divides by zero there).
 template <bool kAF, typename T>
-__global__ __launch_bounds__(256) void meshft_fwd_kernel(
-    const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi, int64_t n, int nfrb,
-    int levels, int parts, T* __restrict__ out /* [n][2] */, double2* __restrict__ part,
-    const double* __restrict__ S_in, float regint) {
-  constexpr int kAcc = kAF ? 4 : 2;  // FP64 sums per lane
-  constexpr size_t kTileBytes = sizeof(FaceRec) * kFwdWaves * 64;
-  constexpr size_t kRedBytes = sizeof(double) * (kFwdWaves - 1) * kAcc * 64;
-  __shared__ __align__(16) unsigned char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
-  // the wave index as a scalar (loop bounds and tile address stay scalar)
-  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-  const int lane = (int)threadIdx.x & 63;
-  FaceRec* tile = reinterpret_cast<FaceRec*>(smem) + wv * 64;
-  int fr;
-  const FwdPiece pc = fwd_piece((int)blockIdx.x, nf, nfrb, levels, parts, fr);
-  const int64_t m = (int64_t)fr * 64 + lane;
-  const bool active = m < n;
-  double X = 0.0, Y = 0.0, Z = 0.0;  // the band frequency, exact (float or double input)
-  if (active) {
-    X = xi[3 * m];
-    Y = xi[3 * m + 1];
-    Z = xi[3 * m + 2];
+__global__ __launch_bounds__(64) void meshft_fwd_kernel(
+    const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi, int64_t n,
+    T* __restrict__ out /* [n][2] */, const double* __restrict__ S_in,
+    const int* __restrict__ e2bits, float regint) {
+  constexpr int kV = FwdLds<kAF>::kV;
+  constexpr int kF = kFwdFreqs<kAF>;
+  __shared__ FwdLds<kAF> sm;
+  const int lane = (int)threadIdx.x;
+  const int64_t m0 = (int64_t)blockIdx.x * kF;
+  const int cntf = (int)((n - m0 < kF) ? n - m0 : kF);
+  bool same_row = false;
+  float xx = 0.0f;
+  if (lane < cntf) {
+    const int64_t m = m0 + lane;
+    const T* xm = xi + 3 * m;
+    FreqRec fr;
+    fr.X0 = xm[0]; fr.X1 = xm[1]; fr.X2 = xm[2];
+    fr.x[0] = (float)fr.X0; fr.x[1] = (float)fr.X1; fr.x[2] = (float)fr.X2;
+    fr.gr = fr.gi = 0.0f;
+    fr.inv2 = 0.0f;
+    sm.rec[lane] = fr;
+    xx = fr.x[0] * fr.x[0] + fr.x[1] * fr.x[1] + fr.x[2] * fr.x[2];
+    same_row = lane > 0 && xm[-3] == xm[0] && xm[-2] == xm[1];
   }
-  const float x = (float)X, y = (float)Y, z = (float)Z;
-  const float x2max = wave_max64(x * x + y * y + z * z);
-  const int64_t g0 = pc.f0 + (pc.f1 - pc.f0) * wv / kFwdWaves;
-  const int64_t g1 = pc.f0 + (pc.f1 - pc.f0) * (wv + 1) / kFwdWaves;
-  double acc[kAcc];
+  const uint64_t cont = __builtin_amdgcn_ballot_w64(same_row);
+  const bool may_be_wide = wave_max64(xx) * __int_as_float(e2bits[0]) > kWidePhase2;
 #pragma unroll
-  for (int i = 0; i < kAcc; ++i) acc[i] = 0.0;
-  for (int64_t base = g0; base < g1; base += 64) {
-    const int cnt = (int)((g1 - base < 64) ? g1 - base : 64);
-    bool wide = false;
-    if (lane < cnt) {
-      const FaceRec r = faces[base + lane];
-      tile[lane] = r;
-      wide = face_extent2(r.eb[0], r.eb[1], r.eb[2], r.ec[0], r.ec[1], r.ec[2]) * x2max >
-             kWidePhase2;
+  for (int i = 0; i < kV * kF; ++i) sm.acc[i][lane] = 0.0;
+  wave_lds_sync();
+  float acc[kV][kF];
+#pragma unroll
+  for (int v = 0; v < kV; ++v)
+#pragma unroll
+    for (int k = 0; k < kF; ++k) acc[v][k] = 0.0f;
+  const int64_t nblk = (nf + 63) / 64;
+  // face record of block b for this lane (past the last face: the last record
+  // again, given zero weight when it is used, so every branch stays finite);
+  // the next block's record is loaded while the current one is evaluated
+  auto load_face = [&](int64_t b) {
+    const int64_t t = b * 64 + lane;
+    return faces[t < nf ? t : nf - 1];
+  };
+  FaceRec nxt = load_face(0);
+  for (int64_t b = 0; b < nblk; ++b) {
+    // the frequency records are re-read from LDS per face block: without this
+    // compiler barrier they would be hoisted into ~80 registers for the launch
+    asm volatile("" ::: "memory");
+    FaceRec r = nxt;
+    nxt = load_face(b + 1 < nblk ? b + 1 : b);  // unconditional: no register copies
+    if (b * 64 + lane >= nf) {
+      r.w = 0.0f;
+      r.an[0] = r.an[1] = r.an[2] = 0.0f;
     }
-    const bool may_be_wide = __builtin_amdgcn_ballot_w64(wide) != 0;
-    wave_lds_sync();
-    fwd_tile<kAF>(tile, cnt, may_be_wide, x, y, z, X, Y, Z, acc[0], acc[1], acc[kAcc - 2],
-                  acc[kAcc - 1]);
-    wave_lds_sync();
-  }
-  // add the waves' sums in a fixed order (the tiles' LDS is reused)
-  __syncthreads();
-  double* red = reinterpret_cast<double*>(smem);
-  if (wv > 0) {
+    double row = 0.0;
+    float prow = 0.0f, qrow = 0.0f, nrow = 0.0f;
 #pragma unroll
-    for (int i = 0; i < kAcc; ++i) red[((wv - 1) * kAcc + i) * 64 + lane] = acc[i];
-  }
-  __syncthreads();
-  if (wv != 0 || !active) return;
+    for (int k = 0; k < kF; ++k) {
+      if (k < cntf) {  // wave-uniform
+        const FreqRec& fr = sm.rec[k];
+        if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
+          row = fma(r.a[1], fr.X1, r.a[0] * fr.X0);
+          prow = __builtin_fmaf(r.eb[1], fr.x[1], r.eb[0] * fr.x[0]);
+          qrow = __builtin_fmaf(r.ec[1], fr.x[1], r.ec[0] * fr.x[0]);
+          if (kAF) nrow = __builtin_fmaf(r.an[1], fr.x[1], r.an[0] * fr.x[0]);
+        }
+        // the same operations, in the same order, as a full evaluation
+        const double rev_a = fma(r.a[2], fr.X2, row);
+        const float p = __builtin_fmaf(r.eb[2], fr.x[2], prow);
+        const float q = __builtin_fmaf(r.ec[2], fr.x[2], qrow);
+        const Sorted so = sort3(p, q);
+        cf F;  // F = e_mid N, face value = -2 m F (r.w carries the -2)
+        float m;
+        if (!may_be_wide || !wave_is_wide(so.D))
+          F = cmul(expmi_rev(mid_rev<false>(rev_a, so.mid)), face_H<false>(so, m));
+        else
+          F = cmul(expmi_rev(mid_rev<true>(rev_a, so.mid)), face_H<true>(so, m));
+        const float wm = r.w * m;
+        acc[0][k] = __builtin_fmaf(wm, F.re, acc[0][k]);
+        acc[1][k] = __builtin_fmaf(wm, F.im, acc[1][k]);
+        if (kAF) {
+          const float tm = __builtin_fmaf(r.an[2], fr.x[2], nrow) * m;
+          acc[kV - 2][k] = __builtin_fmaf(tm, F.re, acc[kV - 2][k]);
+          acc[kV - 1][k] = __builtin_fmaf(tm, F.im, acc[kV - 1][k]);
+        }
+      }
+    }
+    if ((b + 1) % kFwdFold == 0 || b + 1 == nblk) {  // fold into this lane's FP64 slots
 #pragma unroll
-  for (int i = 0; i < kAcc; ++i) {
+      for (int v = 0; v < kV; ++v)
 #pragma unroll
-    for (int w = 0; w < kFwdWaves - 1; ++w) acc[i] += red[(w * kAcc + i) * 64 + lane];
+        for (int k = 0; k < kF; ++k) {
+          sm.acc[v * kF + k][lane] += (double)acc[v][k];
+          acc[v][k] = 0.0f;
+        }
+    }
   }
-  double accr = acc[0], acci = acc[1];
+  // add the 64 lanes' sums of each of the kV * kF values in a fixed order:
+  // with R = 64 / (kV kF) lanes per value, lane l sums value l % (kV kF) over
+  // lanes [j 64/R, (j+1) 64/R), j = l / (kV kF); then lane v adds the R
+  // partials of value v.
+  constexpr int kVals = kV * kF;
+  constexpr int kR = 64 / kVals;
+  static_assert(kVals * kR == 64, "lane reduction layout");
+  wave_lds_sync();
+  const int val = lane % kVals, j = lane / kVals;
+  double s = 0.0;
+#pragma unroll 8
+  for (int l = j * (64 / kR); l < (j + 1) * (64 / kR); ++l) s += sm.acc[val][l];
+  wave_lds_sync();
+  sm.acc[val][j] = s;  // slots [val][0..kR) were read above by this same lane group only
+  wave_lds_sync();
+  if (lane >= kF) return;
+  double tot[kV];
+#pragma unroll
+  for (int v = 0; v < kV; ++v) {
+    double u = 0.0;
+#pragma unroll
+    for (int q = 0; q < kR; ++q) u += sm.acc[v * kF + lane][q];
+    tot[v] = u;
+  }
+  if (lane >= cntf) return;
+  const int64_t m = m0 + lane;
+  double accr = tot[0], acci = tot[1];
   if (kAF) {
-    const double acnr = acc[2], acni = acc[3];
-    const double q = X * X + Y * Y + Z * Z;
+    const FreqRec& fr = sm.rec[lane];
+    const double q = fr.X0 * fr.X0 + fr.X1 * fr.X1 + fr.X2 * fr.X2;
     if (q > 0.0) {
-      accr -= acni / q;
-      acci += acnr / q;
-    } else if (pc.slot == 0 && S_in[1] != 0.0) {
+      accr -= tot[kV - 1] / q;
+      acci += tot[kV - 2] / q;
+    } else if (S_in[1] != 0.0) {
       accr -= (double)regint;
     }
   }
-  if (part) {
-    part[(int64_t)pc.slot * n + m] = make_double2(accr, acci);
-  } else {
-    out[2 * m] = (T)accr;
-    out[2 * m + 1] = (T)acci;
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void fwd_split_reduce_kernel(const double2* __restrict__ part,
-                                                               int nsplit, int64_t n,
-                                                               T* __restrict__ out) {
-  const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (m >= n) return;
-  double re = 0.0, im = 0.0;
-  for (int s = 0; s < nsplit; ++s) {
-    const double2 v = part[(int64_t)s * n + m];
-    re += v.x;
-    im += v.y;
-  }
-  out[2 * m] = (T)re;
-  out[2 * m + 1] = (T)im;
+  out[2 * m] = (T)accr;
+  out[2 * m + 1] = (T)acci;
 }
 
 // -------------------------------------------------------------------- adjoint
@@ -434,8 +424,9 @@ struct FaceLane {
 
 // One LDS tile of the adjoint: this lane's face against cnt frequencies,
 // adding to the face's K sums.  Bit k of `cont` is set when record k lies on
-// the same band row as record k - 1 (same xi_0 and xi_1), so the vertex
-// phase's FP64 row term A0 X0 + A1 X1 is reused; bit 0 is never set.
+// the same band row as record k - 1 (same xi_0 and xi_1), so the row terms of
+// the vertex phase (FP64 A0 X0 + A1 X1) and of the edge phases (b0 x0 + b1 x1,
+// c0 x0 + c1 x1) are reused; bit 0 is never set.
 // may_be_wide: some pair of the tile may have a phase gap above kWidePhase
 // (then every pair takes the wave-wide check of wave_is_wide).
 template <bool kAF>
@@ -446,15 +437,20 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
   float a[K];
 #pragma unroll
   for (int i = 0; i < K; ++i) a[i] = 0.0f;
-  double row = 0.0;
+  double row = 0.0;    // A0 X0 + A1 X1 of the current band row
+  float prow = 0.0f;   // b0 x0 + b1 x1
+  float qrow = 0.0f;   // c0 x0 + c1 x1
   for (int k = 0; k < cnt; ++k) {
     const FreqRec& fr = tile[k];
-    if (!((cont >> k) & 1)) row = fma(L.A1, fr.X1, L.A0 * fr.X0);  // wave-uniform
+    if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
+      row = fma(L.A1, fr.X1, L.A0 * fr.X0);
+      prow = __builtin_fmaf(L.b1, fr.x[1], L.b0 * fr.x[0]);
+      qrow = __builtin_fmaf(L.c1, fr.x[1], L.c0 * fr.x[0]);
+    }
+    // the same operations, in the same order, as a full evaluation
     const double rev_a = fma(L.A2, fr.X2, row);
-    const float p =
-        __builtin_fmaf(L.b2, fr.x[2], __builtin_fmaf(L.b1, fr.x[1], L.b0 * fr.x[0]));
-    const float q =
-        __builtin_fmaf(L.c2, fr.x[2], __builtin_fmaf(L.c1, fr.x[1], L.c0 * fr.x[0]));
+    const float p = __builtin_fmaf(L.b2, fr.x[2], prow);
+    const float q = __builtin_fmaf(L.c2, fr.x[2], qrow);
     const Sorted so = sort3(p, q);
     cf em, d, pP, pQ;
     float rD;
@@ -906,42 +902,6 @@ static int resident_wgs(const void* kernel, int fallback_per_cu) {
   return v;
 }
 
-// Geometric face levels of the forward (DMIC_FWD_LEVELS overrides).
-static int fwd_levels() {
-  static const int r = [] {
-    const char* e = getenv("DMIC_FWD_LEVELS");
-    const int v = e ? atoi(e) : -1;
-    return (v >= 0 && v <= 20) ? v : 4;
-  }();
-  return r;
-}
-
-struct FwdPlan {
-  int nfrb;    // frequency blocks of 64
-  int levels;  // 0: one piece per frequency block, no partials
-  int parts;
-  int nsplit;  // pieces per frequency block (partial slots when > 1)
-};
-
-// part_cap (>= 0) bounds the partials to part_cap double2 entries (nsplit * n):
-// fewer levels, or one piece, when they do not fit.
-static FwdPlan fwd_plan(int64_t nf, int64_t n, int64_t part_cap = -1) {
-  FwdPlan p{(int)ceil_div(n > 0 ? n : 1, 64), 0, 1, 1};
-  if (nf <= 0 || n <= 0) return p;
-  const int cap = resident_wgs((const void*)meshft_fwd_kernel<false, float>, 8);
-  if ((int64_t)p.nfrb >= 32 * (int64_t)cap) return p;  // the band alone fills 32 rounds
-  constexpr int64_t kMinPiece = 64;                     // faces per piece (16 per wave)
-  const int64_t parts = (cap + (int64_t)p.nfrb - 1) / p.nfrb;
-  int levels = fwd_levels();
-  while (levels > 0 && (nf >> (levels > 1 ? levels - 1 : 0)) / parts < kMinPiece) --levels;
-  while (levels > 0 && part_cap >= 0 && (int64_t)levels * parts * n > part_cap) --levels;
-  if (levels == 0 || levels * parts < 2) return p;
-  p.levels = levels;
-  p.parts = (int)parts;
-  p.nsplit = levels * (int)parts;
-  return p;
-}
-
 // Geometric levels per XCD eighth of the band (DMIC_BWD_LEVELS overrides).
 static int bwd_levels() {
   static const int r = [] {
@@ -988,23 +948,20 @@ static BwdPlan bwd_plan(int64_t nf, int64_t n) {
 }
 
 struct FwdWs {
-  int* flag;
+  int* flag;      // [0] face index out of range, [1] largest squared face extent (float bits)
   double* cA;
   double* S;
   double* spart;  // per-workgroup partials of the area and volume sums
   FaceRec* recs;
-  double2* part;
 };
 
-static FwdWs carve_fwd(Carver& c, int64_t nf, int64_t n, int64_t part_cap = -1) {
+static FwdWs carve_fwd(Carver& c, int64_t nf) {
   FwdWs w;
   w.flag = c.take<int>(4);
   w.cA = c.take<double>(nf > 0 ? nf : 1);
   w.S = c.take<double>(2);
   w.spart = c.take<double>(2 * (size_t)ceil_div(nf > 0 ? nf : 1, 256));
   w.recs = c.take<FaceRec>(nf > 0 ? nf : 1);
-  const FwdPlan p = fwd_plan(nf, n, part_cap);
-  w.part = (p.nsplit > 1) ? c.take<double2>((size_t)p.nsplit * n) : nullptr;
   return w;
 }
 
@@ -1074,9 +1031,9 @@ static int prep_faces(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const 
 template <typename IdxT, bool kAF, typename T>
 static int forward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const T* coeff,
                         float regint, const T* xi, int64_t n, T* out, void* ws,
-                        size_t ws_bytes, hipStream_t st, int64_t part_cap = -1) {
+                        size_t ws_bytes, hipStream_t st) {
   Carver c(ws);
-  FwdWs w = carve_fwd(c, nf, n, part_cap);
+  FwdWs w = carve_fwd(c, nf);
   DMIC_REQUIRE(ws_bytes >= c.bytes(), DMIC_ERR_WORKSPACE,
                "forward workspace too small: %zu < %zu bytes", ws_bytes, c.bytes());
   hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, w.flag, 4, 0);
@@ -1092,21 +1049,14 @@ static int forward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, cons
   int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S,
                                w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
-  const FwdPlan p = fwd_plan(nf, n, part_cap);
-  const dim3 grid((unsigned)((int64_t)p.nfrb * p.nsplit));
+  const dim3 grid(ceil_div(n, kFwdFreqs<kAF>));
   rc = timer_begin(0, st);
   if (rc != DMIC_OK) return rc;
-  hipLaunchKernelGGL((meshft_fwd_kernel<kAF, T>), grid, dim3(256), 0, st, w.recs, nf, xi, n,
-                     p.nfrb, p.levels, p.parts, out, p.nsplit > 1 ? w.part : nullptr, w.S,
-                     regint);
+  hipLaunchKernelGGL((meshft_fwd_kernel<kAF, T>), grid, dim3(64), 0, st, w.recs, nf, xi, n, out,
+                     w.S, w.flag + 1, regint);
   DMIC_LAUNCH_CHECK();
   rc = timer_end(0, st);
   if (rc != DMIC_OK) return rc;
-  if (p.nsplit > 1) {
-    hipLaunchKernelGGL(fwd_split_reduce_kernel<T>, dim3(ceil_div(n, 256)), dim3(256), 0, st,
-                       w.part, p.nsplit, n, out);
-    DMIC_LAUNCH_CHECK();
-  }
   return DMIC_OK;
 }
 
@@ -1193,8 +1143,9 @@ const char* dmic_last_error(void) { return g_err; }
 size_t dmic_meshft_forward_workspace(int64_t nv, int64_t nf, int64_t n) {
   (void)nv;
   if (nf < 0 || n < 0) return 0;
+  (void)n;
   Carver c(nullptr);
-  carve_fwd(c, nf, n);
+  carve_fwd(c, nf);
   return c.bytes();
 }
 
@@ -1264,8 +1215,7 @@ int backward_entry(const T* verts, int64_t nv, const void* faces, int faces_is_i
 
 // ------------------------------------------------------- no-allocation forward
 // dmic_meshft_forward_noalloc keeps its scratch memory (the packed face
-// records, the area sums and, when they fit, the face-split partials) in this
-// file-scope device array: one copy per GPU, held until the process ends.
+// records and the area sums) in this file-scope device array: one copy per GPU, held until the process ends.
 // Host code never passes `scratch` itself (its host-side name is not a device
 // address); it uses the per-device address from hipGetSymbolAddress.
 constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
@@ -1309,11 +1259,6 @@ int dmic_meshft_forward_noalloc(const float* verts, int64_t nv, const void* face
                "nf = %lld exceeds the %lld faces the no-allocation forward's %zu-byte scratch "
                "holds; use dmic_meshft_forward with a workspace",
                (long long)nf, (long long)kNoAllocMaxFaces, kNoAllocScratchBytes);
-  // the face records first; the split partials only where the rest of the scratch holds them
-  Carver base(nullptr);
-  carve_fwd(base, nf, n, 0);
-  const int64_t part_cap =
-      (int64_t)((kNoAllocScratchBytes - base.bytes() - 256) / sizeof(double2));
   DMIC_REQUIRE(n < (int64_t(1) << 40), DMIC_ERR_ARG, "size too large");
   DMIC_REQUIRE(n == 0 || (xi_list && ftmesh_list), DMIC_ERR_ARG, "null xi_list/ftmesh_list");
   DMIC_REQUIRE(nf == 0 || n == 0 || (verts && faces && faces_coeff), DMIC_ERR_ARG,
@@ -1324,10 +1269,10 @@ int dmic_meshft_forward_noalloc(const float* verts, int64_t nv, const void* face
   if (faces_is_int64)
     return forward_impl<int64_t, false, float>(verts, nv, static_cast<const int64_t*>(faces), nf,
                                                faces_coeff, 0.0f, xi_list, n, ftmesh_list, ws,
-                                               kNoAllocScratchBytes, stream, part_cap);
+                                               kNoAllocScratchBytes, stream);
   return forward_impl<int32_t, false, float>(verts, nv, static_cast<const int32_t*>(faces), nf,
                                              faces_coeff, 0.0f, xi_list, n, ftmesh_list, ws,
-                                             kNoAllocScratchBytes, stream, part_cap);
+                                             kNoAllocScratchBytes, stream);
 }
 
 int dmic_meshft_forward(const float* verts, int64_t nv, const void* faces, int faces_is_int64,
