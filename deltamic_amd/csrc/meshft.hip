@@ -236,13 +236,15 @@ constexpr float kWidePhase2 = kWidePhase * kWidePhase;
 // Forward: one wave per workgroup, on kFwdFreqs<kAF> consecutive band frequencies
 // (their records in LDS, read as broadcasts) and every face, one face per
 // lane: face block b is faces [64 b, 64 b + 64).  Each lane sums w_t F_t of
-// its faces per frequency in FP32 registers and folds the sums into its own
-// FP64 slots in LDS every kFwdFold face blocks; at the end the 64 lanes'
-// FP64 sums are added in a fixed order.  Consecutive band entries mostly lie
-// on one band row (equal xi_0, xi_1), so the row terms of the vertex phase
-// (FP64) and the edge phases are formed once per row and face.
+// its faces per frequency in FP32 registers; every kFwdFold face blocks and at
+// the end each sum is added over the 64 lanes (a fixed butterfly) and folded
+// into an FP64 total.  No LDS beyond the frequency records, so the kernel runs
+// at 7 waves per SIMD (with FP64 sums kept per lane in LDS it ran at 5, and 6%
+// slower).  Consecutive band entries mostly lie on one band row (equal xi_0,
+// xi_1), so the row terms of the vertex phase (FP64) and of the edge phases
+// are formed once per row and face.
 // 8 frequencies per wave; 4 for the autofluorescence variant, whose four sums
-// per frequency would otherwise double the LDS and registers per wave
+// per frequency would otherwise double the registers per wave
 template <bool kAF>
 constexpr int kFwdFreqs = kAF ? 4 : 8;
 constexpr int kFwdFold = 64;
@@ -251,7 +253,6 @@ template <bool kAF>
 struct FwdLds {
   static constexpr int kV = kAF ? 4 : 2;  // sums per frequency: F (re, im) [+ (xi . an) F]
   FreqRec rec[kFwdFreqs<kAF>];
-  double acc[kV * kFwdFreqs<kAF>][64];  // per-lane FP64 sums; reused for the lane reduction
 };
 
 // kAF: autofluorescence variant (fourier_from_mesh_narrowband_autofluorescence,
@@ -286,9 +287,9 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
   }
   const uint64_t cont = __builtin_amdgcn_ballot_w64(same_row);
   const bool may_be_wide = wave_max64(xx) * __int_as_float(e2bits[0]) > kWidePhase2;
-#pragma unroll
-  for (int i = 0; i < kV * kF; ++i) sm.acc[i][lane] = 0.0;
   wave_lds_sync();
+  // lane v * kF + k keeps the FP64 total of sum v of frequency k
+  double tot = 0.0;
   float acc[kV][kF];
 #pragma unroll
   for (int v = 0; v < kV; ++v)
@@ -296,19 +297,16 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
     for (int k = 0; k < kF; ++k) acc[v][k] = 0.0f;
   const int64_t nblk = (nf + 63) / 64;
   // face record of block b for this lane (past the last face: the last record
-  // again, given zero weight when it is used, so every branch stays finite);
-  // the next block's record is loaded while the current one is evaluated
+  // again, given zero weight, so every branch stays finite)
   auto load_face = [&](int64_t b) {
     const int64_t t = b * 64 + lane;
     return faces[t < nf ? t : nf - 1];
   };
-  FaceRec nxt = load_face(0);
   for (int64_t b = 0; b < nblk; ++b) {
     // the frequency records are re-read from LDS per face block: without this
     // compiler barrier they would be hoisted into ~80 registers for the launch
     asm volatile("" ::: "memory");
-    FaceRec r = nxt;
-    nxt = load_face(b + 1 < nblk ? b + 1 : b);  // unconditional: no register copies
+    FaceRec r = load_face(b);
     if (b * 64 + lane >= nf) {
       r.w = 0.0f;
       r.an[0] = r.an[1] = r.an[2] = 0.0f;
@@ -346,49 +344,34 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
         }
       }
     }
-    if ((b + 1) % kFwdFold == 0 || b + 1 == nblk) {  // fold into this lane's FP64 slots
+    if ((b + 1) % kFwdFold == 0 || b + 1 == nblk) {
+      // fold: each FP32 sum is added over the 64 lanes (butterfly, fixed
+      // order) and into its owner lane's FP64 total
 #pragma unroll
       for (int v = 0; v < kV; ++v)
 #pragma unroll
         for (int k = 0; k < kF; ++k) {
-          sm.acc[v * kF + k][lane] += (double)acc[v][k];
+          float x = acc[v][k];
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+          if (lane == v * kF + k) tot += (double)x;
           acc[v][k] = 0.0f;
         }
     }
   }
-  // add the 64 lanes' sums of each of the kV * kF values in a fixed order:
-  // with R = 64 / (kV kF) lanes per value, lane l sums value l % (kV kF) over
-  // lanes [j 64/R, (j+1) 64/R), j = l / (kV kF); then lane v adds the R
-  // partials of value v.
-  constexpr int kVals = kV * kF;
-  constexpr int kR = 64 / kVals;
-  static_assert(kVals * kR == 64, "lane reduction layout");
-  wave_lds_sync();
-  const int val = lane % kVals, j = lane / kVals;
-  double s = 0.0;
-#pragma unroll 8
-  for (int l = j * (64 / kR); l < (j + 1) * (64 / kR); ++l) s += sm.acc[val][l];
-  wave_lds_sync();
-  sm.acc[val][j] = s;  // slots [val][0..kR) were read above by this same lane group only
-  wave_lds_sync();
-  if (lane >= kF) return;
-  double tot[kV];
+  // lane k gathers the kV totals of frequency k
+  double fin[kV];
 #pragma unroll
-  for (int v = 0; v < kV; ++v) {
-    double u = 0.0;
-#pragma unroll
-    for (int q = 0; q < kR; ++q) u += sm.acc[v * kF + lane][q];
-    tot[v] = u;
-  }
+  for (int v = 0; v < kV; ++v) fin[v] = __shfl(tot, v * kF + (lane & (kF - 1)));
   if (lane >= cntf) return;
   const int64_t m = m0 + lane;
-  double accr = tot[0], acci = tot[1];
+  double accr = fin[0], acci = fin[1];
   if (kAF) {
     const FreqRec& fr = sm.rec[lane];
     const double q = fr.X0 * fr.X0 + fr.X1 * fr.X1 + fr.X2 * fr.X2;
     if (q > 0.0) {
-      accr -= tot[kV - 1] / q;
-      acci += tot[kV - 2] / q;
+      accr -= fin[kV - 1] / q;
+      acci += fin[kV - 2] / q;
     } else if (S_in[1] != 0.0) {
       accr -= (double)regint;
     }
