@@ -2,10 +2,11 @@
 
 ``Fourier3dMesh``, ``render_image_from_ftmesh``, ``normalize_tensor`` and
 ``compute_box_size`` keep the reference's names, argument order, defaults and
-semantics.  The transform itself runs only as HIP kernels on the GPU: the
-reference's dense-torch CPU path (``MeshFTPy``, render.py:36-109, whose
-[N, F] complex intermediates need ~133 GB at 20k faces in a 200^3 box) is not
-provided, and ``device='cpu'`` raises instead of falling back.
+semantics.  On a GPU device the transform runs as HIP kernels (hip_class.py);
+``device='cpu'`` is the reference's CPU device (``MeshFTPy``, render.py:36-109)
+rebuilt in cpu_path.py with torch on the host.  A GPU module never falls back
+to the CPU path: its boundary raises when its inputs are not on the GPU
+(fourier3d_cpp.py, as fourier3d_cuda.cpp:23-25).
 """
 from __future__ import annotations
 
@@ -14,6 +15,7 @@ import torch
 from torch import nn
 
 from . import _lib
+from .cpu_path import meshft_cpu
 from .hip_class import MeshFTAutofluorescenceHIP, MeshFTHIP, clear_band_cache
 
 
@@ -165,9 +167,10 @@ class Fourier3dMesh(nn.Module):
         """Verts [nv, 3] float32, Faces [nf, 3] int64/int32, Faces_coeff [nf]
         -> complex64 tensor of shape box_shape (zero outside the band)."""
         if torch.device(self.device).type == 'cpu':
-            raise NotImplementedError(
-                "deltamic_amd computes meshFT with HIP kernels only; construct "
-                "Fourier3dMesh with a GPU device (e.g. device='cuda:0').")
+            # the reference's CPU device (MeshFTPy, render.py:142-144): torch on
+            # the host, see cpu_path.py; GPU modules never take this branch
+            return meshft_cpu(Verts - self.box_size[:, 0], Faces, Faces_coeff, self.xi0,
+                              self.xi1, self.xi2, self.OTF, self.narrowband_thresh)
         return MeshFTHIP.apply(Verts - self.box_size[:, 0], Faces, Faces_coeff, self.xi0,
                                self.xi1, self.xi2, self.OTF, self.narrowband_thresh,
                                self.cache_band)

@@ -129,14 +129,6 @@ def test_frequency_axes_bit_identical_to_restatement():
     assert m.narrowband_thresh == 0.01
 
 
-def test_cpu_device_raises_instead_of_falling_back():
-    from deltamic_amd import Fourier3dMesh
-    w = syn.config1()
-    m = Fourier3dMesh(torch.tensor(w.box_size), (8, 8, 8), device="cpu")
-    with pytest.raises(NotImplementedError):
-        m(torch.tensor(w.verts), torch.tensor(w.faces), torch.ones(len(w.faces)))
-
-
 def test_native_boundary_checks_like_reference():
     from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
     V = torch.zeros(3, 3)
