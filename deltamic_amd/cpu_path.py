@@ -106,25 +106,54 @@ def _chunks(n: int, nf: int):
     return [(s, min(n, s + step)) for s in range(0, n, step)]
 
 
-def band_values(V: torch.Tensor, F: torch.Tensor, C: torch.Tensor, xi: torch.Tensor) -> torch.Tensor:
-    """ftmesh on the band frequencies xi [n, 3]: sum_t w_t G_t (complex128 [n])."""
+def _chunk_values(V, F, C, xi, regint):
+    """ftmesh on the frequencies xi (complex128 [n]).  regint is None for the
+    surface transform sum_t w_t G_t, w_t = 2 c_t A_t / sum c A; else the
+    autofluorescence transform (render.py:228-318)
+        sum_t G_t (w_t + i (xi . nor2_t) / |xi|^2),  w_t = 2 A_t / sum A,
+        nor2_t = (b - a) x (c - a) regint / regvol,  regvol = -sum_t a . (b x c) / 6,
+    whose interior term at xi = 0 is its limit -regint (the reference divides
+    by zero there)."""
+    G = face_values(V, F, xi)
     w = face_weights(V, F, C).to(torch.complex128)
+    out = G @ w
+    if regint is None:
+        return out
+    P = V[F]
+    nor = torch.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0], dim=1)
+    regvol = -(P[:, 0] * torch.cross(P[:, 1], P[:, 2], dim=1)).sum() / 6.0
+    ok = regvol != 0
+    scale = torch.where(ok, regint / torch.where(ok, regvol, torch.ones_like(regvol)),
+                        torch.zeros_like(regvol))
+    s = (G * (xi @ (nor * scale).T)).sum(1)  # sum_t G_t (xi . nor2_t)
+    q = (xi * xi).sum(1)
+    zero = q == 0
+    inv = torch.where(zero, torch.zeros_like(q), 1.0 / torch.where(zero, torch.ones_like(q), q))
+    lim = torch.where(zero & ok, torch.full_like(q, -float(regint)), torch.zeros_like(q))
+    return out + torch.complex(lim, torch.zeros_like(lim)) + 1j * inv * s
+
+
+def band_values(V: torch.Tensor, F: torch.Tensor, C: torch.Tensor, xi: torch.Tensor,
+                regint=None) -> torch.Tensor:
+    """ftmesh on the band frequencies xi [n, 3] (complex128 [n]), chunk by chunk."""
     out = torch.empty(xi.shape[0], dtype=torch.complex128)
     for s, e in _chunks(xi.shape[0], F.shape[0]):
-        out[s:e] = face_values(V, F, xi[s:e]) @ w
+        out[s:e] = _chunk_values(V, F, C, xi[s:e], regint)
     return out
 
 
 class MeshFTCPU(torch.autograd.Function):
     """ftmesh on the band (complex128 [n]) from FP64 vertices, differentiable
     w.r.t. the vertices; the backward re-evaluates one band chunk at a time
-    under autograd (no [band, faces] graph is kept)."""
+    under autograd (no [band, faces] graph is kept).  regint: None for the
+    surface transform, a number for the autofluorescence variant."""
 
     @staticmethod
-    def forward(ctx, V, F, C, xi):
+    def forward(ctx, V, F, C, xi, regint=None):
         ctx.save_for_backward(V, F, C, xi)
+        ctx.regint = regint
         with torch.no_grad():
-            return band_values(V, F, C, xi)
+            return band_values(V, F, C, xi, regint)
 
     @staticmethod
     def backward(ctx, g):
@@ -134,11 +163,10 @@ class MeshFTCPU(torch.autograd.Function):
         for s, e in _chunks(xi.shape[0], F.shape[0]):
             with torch.enable_grad():
                 Vl = V.detach().requires_grad_(True)
-                w = face_weights(Vl, F, C).to(torch.complex128)
-                val = face_values(Vl, F, xi[s:e]) @ w
+                val = _chunk_values(Vl, F, C, xi[s:e], ctx.regint)
                 (gv,) = torch.autograd.grad(val, Vl, g[s:e])
             grad += gv
-        return grad, None, None, None
+        return grad, None, None, None, None
 
 
 def meshft_cpu(Verts, Faces, Faces_coeff, xi0, xi1, xi2, OTF, narrowband_thresh):
@@ -153,3 +181,19 @@ def meshft_cpu(Verts, Faces, Faces_coeff, xi0, xi1, xi2, OTF, narrowband_thresh)
     out = torch.zeros(tuple(mask.shape), dtype=ctype)
     out = out.index_put((mask,), vals.to(ctype))
     return out
+
+
+def meshft_af_cpu(OTF, narrowband_thresh, regint, xi0, xi1, xi2, Verts, Faces):
+    """``fourier_from_mesh_narrowband_autofluorescence`` on host tensors
+    (render.py:228-318): band ``OTF > thresh`` (this function of the reference
+    takes no abs), phases of the given vertices, complex box of the axes'
+    complex type, zero outside the band."""
+    mask = OTF > narrowband_thresh
+    g0, g1, g2 = torch.meshgrid(xi0, xi1, xi2, indexing="ij")
+    xi = torch.stack((g0[mask], g1[mask], g2[mask]), dim=1).to(torch.float64)
+    F = Faces.long()
+    vals = MeshFTCPU.apply(Verts.to(torch.float64), F,
+                           torch.ones(F.shape[0], dtype=torch.float64), xi, float(regint))
+    ctype = torch.complex128 if xi0.dtype == torch.float64 else torch.complex64
+    out = torch.zeros(tuple(mask.shape), dtype=ctype)
+    return out.index_put((mask,), vals.to(ctype))

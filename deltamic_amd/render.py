@@ -15,7 +15,7 @@ import torch
 from torch import nn
 
 from . import _lib
-from .cpu_path import meshft_cpu
+from .cpu_path import meshft_af_cpu, meshft_cpu
 from .hip_class import MeshFTAutofluorescenceHIP, MeshFTHIP, clear_band_cache
 
 
@@ -265,9 +265,11 @@ def fourier_from_mesh_narrowband_autofluorescence(OTF, narrowband_thresh, regint
     symmetric bands) and differentiable w.r.t. Verts; the reference evaluates
     the dense [band, faces] expression with torch.  Differences: areas by
     the cross product instead of Heron's formula, and the interior term at
-    xi = 0 is its limit -regint instead of 0/0."""
+    xi = 0 is its limit -regint instead of 0/0.  Host tensors take the torch
+    path of cpu_path.py (FP64 arithmetic, chunked autograd)."""
     if not Verts.is_cuda:
-        raise NotImplementedError("deltamic_amd computes meshFT with HIP kernels only; "
-                                  "pass CUDA (HIP) tensors")
+        # host tensors: the torch path of cpu_path.py (the reference's function
+        # is device-agnostic torch)
+        return meshft_af_cpu(OTF, narrowband_thresh, regint, xi0, xi1, xi2, Verts, Faces)
     return MeshFTAutofluorescenceHIP.apply(Verts, Faces, regint, xi0, xi1, xi2, OTF,
                                            narrowband_thresh)

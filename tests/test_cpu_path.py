@@ -120,3 +120,60 @@ def test_band_is_abs_otf_above_threshold_and_degenerate_faces_stay_finite():
     assert O.rel_l2(val.detach().numpy(), ref) <= 1e-12
     val.real.sum().backward()
     assert torch.isfinite(Vt.grad).all()
+
+
+def test_autofluorescence_unit_cube_vs_exact_transform():
+    """Surface minus regint times the solid cube's transform (the interior
+    term at xi = 0 is its limit -regint)."""
+    c = load("cube_surface.json")
+    V = torch.tensor(c["verts"], dtype=torch.float64)
+    F = torch.tensor(c["faces"])
+    xi = torch.tensor(c["xi"], dtype=torch.float64)
+    got = band_values(V, F, torch.ones(len(F), dtype=torch.float64), xi, regint=0.7).numpy()
+    want = cplx(c["surface"]) - 0.7 * cplx(c["solid"])
+    want[np.all(np.array(c["xi"]) == 0, axis=1)] = 1.0 - 0.7
+    assert np.max(np.abs(got - want)) <= 1e-10
+
+
+def test_autofluorescence_on_host_vs_oracle_and_differences():
+    """fourier_from_mesh_narrowband_autofluorescence on host tensors: band
+    values against the FP64 oracle (config 1), zeros outside OTF > thresh, and
+    Verts.grad from autograd against central differences of the same FP64
+    function on seeded band frequencies and vertices."""
+    from deltamic_amd import fourier_from_mesh_narrowband_autofluorescence
+    w, m = _config1_module(torch.float64)
+    V = torch.tensor(w.verts, dtype=torch.float64)
+    F = torch.tensor(w.faces)
+    box = fourier_from_mesh_narrowband_autofluorescence(m.OTF, w.thresh, 0.5, m.xi0, m.xi1,
+                                                        m.xi2, V, F)
+    mask = m.OTF > w.thresh
+    assert box.dtype == torch.complex128 and bool((box[~mask] == 0).all())
+    g0, g1, g2 = torch.meshgrid(m.xi0, m.xi1, m.xi2, indexing="ij")
+    xi = torch.stack((g0[mask], g1[mask], g2[mask]), 1)
+    ref = O.acc_forward_af(w.verts, w.faces, np.ones(len(w.faces)), 0.5, xi.numpy(),
+                           fp32_inputs=False)
+    assert O.rel_l2(box[mask].numpy(), ref) <= 1e-11
+    r = syn.rng(w.seed + 11)
+    sel = torch.tensor(np.sort(r.choice(len(xi), 256, replace=False)))
+    xs = xi[sel]
+    g = torch.tensor(r.standard_normal(256) + 1j * r.standard_normal(256))
+    ones = torch.ones(len(F), dtype=torch.float64)
+
+    def loss(Vx):
+        val = MeshFTCPU.apply(Vx, F, ones, xs, 0.5)
+        return (val.real * g.real + val.imag * g.imag).sum()
+
+    Vg = V.clone().requires_grad_(True)
+    loss(Vg).backward()
+    h = 1e-6
+    verts = r.choice(len(w.verts), 6, replace=False)
+    fd, ad = [], []
+    with torch.no_grad():
+        for i in verts:
+            for d in range(3):
+                Vp, Vm = V.clone(), V.clone()
+                Vp[i, d] += h
+                Vm[i, d] -= h
+                fd.append(float(loss(Vp) - loss(Vm)) / (2 * h))
+                ad.append(float(Vg.grad[i, d]))
+    assert np.linalg.norm(np.array(ad) - fd) <= 1e-6 * np.linalg.norm(fd)
