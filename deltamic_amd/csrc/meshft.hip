@@ -1198,11 +1198,12 @@ int backward_entry(const T* verts, int64_t nv, const void* faces, int faces_is_i
 
 // ------------------------------------------------------- no-allocation forward
 // dmic_meshft_forward_noalloc keeps its scratch memory (the packed face
-// records and the area sums) in this file-scope device array: one copy per GPU, held until the process ends.
+// records and the area sums) in this file-scope device array: one copy per
+// GPU, held until the process ends.
 // Host code never passes `scratch` itself (its host-side name is not a device
 // address); it uses the per-device address from hipGetSymbolAddress.
-constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
-constexpr int64_t kNoAllocMaxFaces = 262144;  // 72 B of records per face: 18.9 MB
+constexpr size_t kNoAllocScratchBytes = size_t(20) << 20;
+constexpr int64_t kNoAllocMaxFaces = 262144;  // 72 B per face: 18.9 MB + the area-sum partials
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {

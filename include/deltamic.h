@@ -86,10 +86,9 @@ int dmic_meshft_backward_f64(const double* verts, int64_t nv, const void* faces,
  * counterpart in the reference: it is for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  Its scratch memory (packed face records
- * and area sums; the face-split partials when they fit) is a static 32 MiB
- * device array that each GPU the library runs on holds until the process
- * ends.  Limit: nf <= 262144 faces (DMIC_ERR_ARG beyond); any band size n
- * (large bands are evaluated without the face split).  Every call on one GPU
+ * and area sums, 72 B per face) is a static 20 MiB device array that each
+ * GPU the library runs on holds until the process ends.  Limit:
+ * nf <= 262144 faces (DMIC_ERR_ARG beyond); any band size n.  Every call on one GPU
  * shares that GPU's array, so calls that overlap on one GPU (other streams,
  * other threads) race and must be serialised by the caller.  A face index
  * outside [0, nv) reads as the origin and is not reported (there is no
