@@ -61,7 +61,6 @@ struct __align__(16) FreqRec {
 };
 static_assert(sizeof(FreqRec) == 48, "FreqRec layout");
 
-constexpr int kFold = 64;  // FP32 partial sums fold into FP64 every kFold pairs
 
 // -------------------------------------------------------------------- prep
 
@@ -412,6 +411,8 @@ struct FaceLane {
 // c0 x0 + c1 x1) are reused; bit 0 is never set.
 // may_be_wide: some pair of the tile may have a phase gap above kWidePhase
 // (then every pair takes the wave-wide check of wave_is_wide).
+// The FP32 sums of a tile (at most 64 frequencies) are folded into the FP64
+// sums at its end.
 template <bool kAF>
 __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t cont,
                                          bool may_be_wide, const FaceLane& L,
