@@ -533,3 +533,30 @@ def test_autofluorescence_module_function(device):
     err = O.rel_l2(V.grad.cpu().numpy(), ref.cpu().numpy())
     print(f"autofluorescence module gradient vs direct adjoint: {err:.2e}")
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("nf,n", [(1, 1), (63, 7), (64, 8), (65, 9), (65, 300), (700, 5000),
+                                  (130, 2049)])
+def test_work_decomposition_sizes(device, nf, n):
+    """Forward and adjoint against the FP64 oracle at sizes on and off the
+    kernels' block boundaries: 64-face blocks (one lane per face), 8-frequency
+    forward workgroups, 64-frequency adjoint tiles and the chunk plans (equal
+    chunks for small bands, geometric levels per XCD eighth for large ones)."""
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    r = syn.rng(7000 + nf * 13 + n)
+    nv = max(3, nf // 2 + 2)
+    V = (0.3 * r.standard_normal((nv, 3)) + 1.0).astype(np.float32)
+    Fc = np.stack([r.permutation(nv)[:3] for _ in range(nf)]).astype(np.int64)
+    C = (0.5 + r.random(nf)).astype(np.float32)
+    xi = (6.0 * r.standard_normal((n, 3))).astype(np.float32)
+    g = (r.standard_normal(n) + 1j * r.standard_normal(n)).astype(np.complex64)
+    Vd, Fd, Cd = (torch.tensor(a, device=device) for a in (V, Fc, C))
+    xid = torch.tensor(xi, device=device)
+    ft = forward_fourier3d_cpp(Vd, Fd, Cd, xid)
+    ref = O.acc_forward(V, Fc, C, xi)
+    ferr = O.rel_l2(ft.cpu().numpy(), ref)
+    gv = backward_fourier3d_cpp(Vd, Fd, Cd, torch.tensor(g, device=device), ft, xid)
+    gref = O.acc_backward(V, Fc, C, xi, g)
+    berr = O.rel_l2(gv.cpu().numpy(), gref)
+    print(f"nf={nf} n={n}: forward rel-L2 {ferr:.2e}, adjoint rel-L2 {berr:.2e}")
+    assert ferr <= FWD_TOL and berr <= BWD_TOL
