@@ -1020,6 +1020,10 @@ static int forward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, cons
   FwdWs w = carve_fwd(c, nf);
   DMIC_REQUIRE(ws_bytes >= c.bytes(), DMIC_ERR_WORKSPACE,
                "forward workspace too small: %zu < %zu bytes", ws_bytes, c.bytes());
+  // one workgroup per kFwdFreqs band entries on a 1-D grid
+  DMIC_REQUIRE(n <= (int64_t)0x7fffffff * kFwdFreqs<kAF>, DMIC_ERR_ARG,
+               "band of %lld frequencies exceeds the forward's grid (%lld)", (long long)n,
+               (long long)0x7fffffff * kFwdFreqs<kAF>);
   hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, w.flag, 4, 0);
   DMIC_LAUNCH_CHECK();
   if (n == 0) return DMIC_OK;
