@@ -102,23 +102,28 @@ __device__ __forceinline__ float mid_rev(double rev_a, float mid) {
   if (kWide) return reduce_rev(fma((double)mid, kInv2Pi, rev_a));
   return __builtin_fmaf(mid, kInv2PiF, reduce_rev(rev_a));
 }
-// phi'(y) = [(-sin y + k s) + i(-cos y + k c)] / y.
-// Real part = k (k/2 - c): no cancellation for any y (k, c -> 1 as y -> 0).
-// Imaginary part = (k c + 2 s^2 - 1)/y (the spherical Bessel j1(y)) cancels
-// for small y (relative error ~5e-7/y^2 from the sin/cos); below kPhiSeries
-// it comes from its series y (1/3 - y^2/30 + y^4/840 - y^6/45360)
-// (truncation < 6e-8 at |y| = 0.85, where the closed form is within 1.5e-6).
-__device__ __forceinline__ cf dphi_of(float y, float s, float c, float k, float ry) {
-  const float re = k * __builtin_fmaf(0.5f, k, -c);
+// phi'(y)/2, from s = sin(y/2), c = cos(y/2), h = s/y (= k/2 with
+// k = sin(y/2)/(y/2)) and ry = 1/y:
+//     phi'(y) = [(-sin y + k s) + i(-cos y + k c)] / y.
+// Real part / 2 = h (h - c): no cancellation for any y (2h, c -> 1 as y -> 0).
+// Imaginary part / 2 = (h c + s^2 - 1/2)/y (half the spherical Bessel j1(y))
+// cancels for small y (relative error ~5e-7/y^2 from the sin/cos); below
+// kPhiSeries it comes from its series y (1/6 - y^2/60 + y^4/1680 - y^6/90720)
+// (truncation < 3e-8 at |y| = 0.85, where the closed form is within 1.5e-6).
+__device__ __forceinline__ cf dphi_half(float y, float s, float c, float h, float ry) {
+  const float re = h * (h - c);
   const float y2 = y * y;
-  float si = __builtin_fmaf(y2, -2.2045855379e-5f, 1.1904761905e-3f);
-  si = __builtin_fmaf(y2, si, -3.3333333333e-2f);
-  si = __builtin_fmaf(y2, si, 3.3333333333e-1f) * y;
-  const float di = __builtin_fmaf(k, c, __builtin_fmaf(2.0f * s, s, -1.0f)) * ry;
+  float si = __builtin_fmaf(y2, -1.1022927690e-5f, 5.9523809524e-4f);
+  si = __builtin_fmaf(y2, si, -1.6666666667e-2f);
+  si = __builtin_fmaf(y2, si, 1.6666666667e-1f) * y;
+  const float di = __builtin_fmaf(h, c, __builtin_fmaf(s, s, -0.5f)) * ry;
   return {re, __builtin_fabsf(y) < kPhiSeries ? si : di};
 }
-// Taylor branch: H(P,Q) = sum_k -(-i)^k h_{k-2}(P,Q)/k!, k = 2..8.
+// Taylor branch: H(P,Q) = sum_k -(-i)^k h_{k-2}(P,Q)/k!, k = 2..8
+// (kHalf: H/2, the same series with every coefficient halved, exactly).
+template <bool kHalf = false>
 __device__ __forceinline__ cf taylor_H(float P, float Q) {
+  constexpr float c = kHalf ? 0.5f : 1.0f;
   const float s1 = P + Q, t1 = P * Q;
   const float h1 = s1;
   const float h2 = s1 * h1 - t1;
@@ -126,23 +131,28 @@ __device__ __forceinline__ cf taylor_H(float P, float Q) {
   const float h4 = s1 * h3 - t1 * h2;
   const float h5 = s1 * h4 - t1 * h3;
   const float h6 = s1 * h5 - t1 * h4;
-  const float re = 0.5f - h2 * 4.1666666667e-2f + h4 * 1.3888888889e-3f - h6 * 2.4801587302e-5f;
-  const float im = -h1 * 1.6666666667e-1f + h3 * 8.3333333333e-3f - h5 * 1.9841269841e-4f;
+  const float re = c * 0.5f - h2 * (c * 4.1666666667e-2f) + h4 * (c * 1.3888888889e-3f) -
+                   h6 * (c * 2.4801587302e-5f);
+  const float im = -h1 * (c * 1.6666666667e-1f) + h3 * (c * 8.3333333333e-3f) -
+                   h5 * (c * 1.9841269841e-4f);
   return {re, im};
 }
 
 // Taylor branch derivative dH/dP = sum_k -(-i)^k g_{k-3}/k!, k = 3..8, where
 // g_j = h_j(P, P, Q) (the derivative of h_j(P,Q) w.r.t. P duplicates P).
-// dH/dQ is the same with P and Q exchanged.
-__device__ __forceinline__ cf taylor_dH(float P, float Q) {
+// dH/dQ is the same with P and Q exchanged.  The adjoint's only caller
+// takes half of it (coefficients halved, exactly).
+__device__ __forceinline__ cf taylor_dH_half(float P, float Q) {
   const float E1 = 2.0f * P + Q, E2 = P * (P + 2.0f * Q), E3 = P * P * Q;
   const float g1 = E1;
   const float g2 = E1 * g1 - E2;
   const float g3 = E1 * g2 - E2 * g1 + E3;
   const float g4 = E1 * g3 - E2 * g2 + E3 * g1;
   const float g5 = E1 * g4 - E2 * g3 + E3 * g2;
-  const float re = -g1 * 4.1666666667e-2f + g3 * 1.3888888889e-3f - g5 * 2.4801587302e-5f;
-  const float im = -1.6666666667e-1f + g2 * 8.3333333333e-3f - g4 * 1.9841269841e-4f;
+  const float re = -g1 * (0.5f * 4.1666666667e-2f) + g3 * (0.5f * 1.3888888889e-3f) -
+                   g5 * (0.5f * 2.4801587302e-5f);
+  const float im = -(0.5f * 1.6666666667e-1f) + g2 * (0.5f * 8.3333333333e-3f) -
+                   g4 * (0.5f * 1.9841269841e-4f);
   return {re, im};
 }
 
@@ -199,18 +209,19 @@ __device__ __forceinline__ cf face_H(const Sorted& so, float& m) {
   }
   return N;
 }
-// H and its derivatives for the adjoint, returned as (d, pP, pQ, rD) with
-//     H = d rD,  dH/dP = (pP + H) rD,  dH/dQ = -(pQ + H) rD,
-// i.e. d = phi(P) - phi(Q), pP = phi'(P), pQ = phi'(Q), rD = 1/D in the
-// closed form (phi(y) = -k (s + i c), k = sin(y/2)/(y/2) = 2 s / y), and
-// d = H, rD = 1 in the Taylor branch.  The kernel contracts with the
-// upstream gradient before scaling by rD.
+// H and its derivatives for the adjoint, returned halved as (d, pP, pQ, rD) with
+//     H/2 = d rD,  (dH/dP)/2 = (pP + d rD) rD,  (dH/dQ)/2 = -(pQ + d rD) rD,
+// i.e. d = (phi(P) - phi(Q))/2, pP = phi'(P)/2, pQ = phi'(Q)/2, rD = 1/D in
+// the closed form (phi(y) = -2 h (s + i c), h = sin(y/2)/y), and d = H/2,
+// rD = 1 in the Taylor branch.  The halving saves the factor 2 of
+// phi = -k (s + i c), k = 2h; the adjoint carries 2 |g| instead of |g|.  The
+// kernel contracts with the upstream gradient before scaling by rD.
 template <bool kWide>
 __device__ __forceinline__ void face_H_grad(const Sorted& so, cf& d, cf& pP, cf& pQ, float& rD) {
   if (so.D < kTaylorD) {
-    d = taylor_H(so.P, so.Q);
-    const cf tP = taylor_dH(so.P, so.Q);
-    const cf tQ = taylor_dH(so.Q, so.P);
+    d = taylor_H<true>(so.P, so.Q);
+    const cf tP = taylor_dH_half(so.P, so.Q);
+    const cf tQ = taylor_dH_half(so.Q, so.P);
     pP = {tP.re - d.re, tP.im - d.im};
     pQ = {-tQ.re - d.re, -tQ.im - d.im};
     rD = 1.0f;
@@ -224,10 +235,10 @@ __device__ __forceinline__ void face_H_grad(const Sorted& so, cf& d, cf& pP, cf&
     const float Dr = so.D * r;
     const float rP = Q * Dr, rQ = P * Dr;   // 1/P, 1/Q
     rD = PQ * r;
-    const float kP = 2.0f * sP * rP, kQ = 2.0f * sQ * rQ;
-    d = {kQ * sQ - kP * sP, kQ * cQ - kP * cP};
-    pP = dphi_of(P, sP, cP, kP, rP);
-    pQ = dphi_of(Q, sQ, cQ, kQ, rQ);
+    const float hP = sP * rP, hQ = sQ * rQ;
+    d = {hQ * sQ - hP * sP, hQ * cQ - hP * cP};
+    pP = dphi_half(P, sP, cP, hP, rP);
+    pQ = dphi_half(Q, sQ, cQ, hQ, rQ);
   }
 }
 }  // namespace dmic

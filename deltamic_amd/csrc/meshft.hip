@@ -55,7 +55,8 @@ static_assert(sizeof(FaceRec) == 64, "FaceRec layout");
 struct __align__(16) FreqRec {
   double X2;      // xi_2 in FP64
   float x[3];     // xi in FP32
-  float gr, gi;   // upstream gradient g = dL/dRe + i dL/dIm
+  float gm, gph;  // upstream gradient g = dL/dRe + i dL/dIm in polar form:
+                  // 2 |g| and arg(g) / (2 pi) in [-1, 0) revolutions
   float inv2;     // autofluorescence variant: 1 / |xi|^2 (0 at xi = 0); else 0
   double X0, X1;  // xi_0, xi_1 in FP64
 };
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
     FreqRec fr;
     fr.X0 = xm[0]; fr.X1 = xm[1]; fr.X2 = xm[2];
     fr.x[0] = (float)fr.X0; fr.x[1] = (float)fr.X1; fr.x[2] = (float)fr.X2;
-    fr.gr = fr.gi = 0.0f;
+    fr.gm = fr.gph = 0.0f;
     fr.inv2 = 0.0f;
     sm.rec[lane] = fr;
     xx = fr.x[0] * fr.x[0] + fr.x[1] * fr.x[1] + fr.x[2] * fr.x[2];
@@ -408,7 +409,11 @@ struct FaceLane {
 // adding to the face's K sums.  Bit k of `cont` is set when record k lies on
 // the same band row as record k - 1 (same xi_0 and xi_1), so the row terms of
 // the vertex phase (FP64 A0 X0 + A1 X1) and of the edge phases (b0 x0 + b1 x1,
-// c0 x0 + c1 x1) are reused; bit 0 is never set.
+// c0 x0 + c1 x1) are reused; bit 0 is never set.  Along a row the xi_0 and
+// xi_1 parts of the vertex sums sum_m (.) xi_m are xi_0 and xi_1 times the
+// row's sum of (.): those are kept per row and added when the row ends
+// (not in the autofluorescence variant, whose extra registers would cost
+// it a wave per SIMD).
 // may_be_wide: some pair of the tile may have a phase gap above kWidePhase
 // (then every pair takes the wave-wide check of wave_is_wide).
 // The FP32 sums of a tile (at most 64 frequencies) are folded into the FP64
@@ -418,53 +423,84 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
                                          bool may_be_wide, const FaceLane& L,
                                          double (&acc)[kFaceSums<kAF>]) {
   constexpr int K = kFaceSums<kAF>;
+  constexpr bool kRowSums = !kAF;
+  // sums with a xi factor: 3 (vertex a, vertex b, all vertices) or 4 (+ N)
+  constexpr int R = kAF ? 4 : 3;
   float a[K];
 #pragma unroll
   for (int i = 0; i < K; ++i) a[i] = 0.0f;
+  float rs[R];  // this row's sums of the xi factors' coefficients
+#pragma unroll
+  for (int i = 0; i < R; ++i) rs[i] = 0.0f;
+  float rx0 = 0.0f, rx1 = 0.0f;  // this row's xi_0, xi_1
+  auto end_row = [&]() {
+    if (!kRowSums) return;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const int o = i < 3 ? 3 * i : 10;
+      a[o] = __builtin_fmaf(rs[i], rx0, a[o]);
+      a[o + 1] = __builtin_fmaf(rs[i], rx1, a[o + 1]);
+      rs[i] = 0.0f;
+    }
+  };
+  // adds v xi to the sums of factor i
+  auto add_xi = [&](int i, float v, const FreqRec& fr) {
+    const int o = i < 3 ? 3 * i : 10;
+    if (kRowSums) {
+      rs[i] += v;
+    } else {
+      a[o] = __builtin_fmaf(v, fr.x[0], a[o]);
+      a[o + 1] = __builtin_fmaf(v, fr.x[1], a[o + 1]);
+    }
+    a[o + 2] = __builtin_fmaf(v, fr.x[2], a[o + 2]);
+  };
   double row = 0.0;    // A0 X0 + A1 X1 of the current band row
   float prow = 0.0f;   // b0 x0 + b1 x1
   float qrow = 0.0f;   // c0 x0 + c1 x1
   for (int k = 0; k < cnt; ++k) {
     const FreqRec& fr = tile[k];
     if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
+      if (kRowSums && k > 0) end_row();
       row = fma(L.A1, fr.X1, L.A0 * fr.X0);
       prow = __builtin_fmaf(L.b1, fr.x[1], L.b0 * fr.x[0]);
       qrow = __builtin_fmaf(L.c1, fr.x[1], L.c0 * fr.x[0]);
+      rx0 = fr.x[0];
+      rx1 = fr.x[1];
     }
     // the same operations, in the same order, as a full evaluation
     const double rev_a = fma(L.A2, fr.X2, row);
     const float p = __builtin_fmaf(L.b2, fr.x[2], prow);
     const float q = __builtin_fmaf(L.c2, fr.x[2], qrow);
     const Sorted so = sort3(p, q);
-    cf em, d, pP, pQ;
-    float rD;
+    cf d, pP, pQ;
+    float rD, u;  // u: phase of the median vertex plus arg(g) / (2 pi), in revolutions
     if (!may_be_wide || !wave_is_wide(so.D)) {
-      em = expmi_rev(mid_rev<false>(rev_a, so.mid));
+      u = __builtin_fmaf(so.mid, kInv2PiF, reduce_rev(rev_a) + fr.gph);
       face_H_grad<false>(so, d, pP, pQ, rD);
     } else {
-      em = expmi_rev(mid_rev<true>(rev_a, so.mid));
+      u = mid_rev<true>(rev_a, so.mid) + fr.gph;
       face_H_grad<true>(so, d, pP, pQ, rD);
     }
-    // Contract with conj(g) first, scaled by rD: z = conj(g) e_mid rD, then
-    // with H = d rD
+    // Contract with conj(g) first, scaled by rD: z = 2 conj(g) e_mid rD
+    // = 2 |g| rD exp(-2 pi i u); with the halved d, pP, pQ (H = 2 d rD)
     //   rho = Re(conj g F) = Re(z d),  gamma = Re(conj g (-iF)) = Im(z d),
     //   Re(conj g dF/dx_lo) = Re(z pP) + rho rD,
     //   Re(conj g dF/dx_hi) = -(Re(z pQ) + rho rD),
     //   Re(conj g dF/dx_mid) = gamma - (lo) - (hi).
-    const cf e = cmul({fr.gr, -fr.gi}, em);
-    const cf z = {e.re * rD, e.im * rD};
+    const cf em = expmi_rev(u);
+    const float zm = fr.gm * rD;
+    const cf z = {zm * em.re, zm * em.im};
     const float rh = __builtin_fmaf(z.re, d.re, -z.im * d.im);
-    cf zt = z;      // conj(g) e_mid rD times the face weight when it sits inside the sum
+    cf zt = z;      // z times the face weight when it sits inside the sum
     float rt = rh;  // Re(zt d)
     if (kAF) {
-      // z2 = i conj(g) e_mid rD / |xi|^2; zt = w z + s z2 with s = xi . an
+      // z2 = i z / |xi|^2; zt = w z + s z2 with s = xi . an
       const cf z2 = {-z.im * fr.inv2, z.re * fr.inv2};
       const float sn = __builtin_fmaf(L.n2, fr.x[2], __builtin_fmaf(L.n1, fr.x[1], L.n0 * fr.x[0]));
       zt = {__builtin_fmaf(L.fw, z.re, sn * z2.re), __builtin_fmaf(L.fw, z.im, sn * z2.im)};
       rt = __builtin_fmaf(zt.re, d.re, -zt.im * d.im);
       const float nn = __builtin_fmaf(z2.re, d.re, -z2.im * d.im);
-#pragma unroll
-      for (int j = 0; j < 3; ++j) a[10 + j] = __builtin_fmaf(nn, fr.x[j], a[10 + j]);
+      add_xi(3, nn, fr);
     }
     const float rtD = rt * rD;
     const float ga = __builtin_fmaf(zt.re, d.im, zt.im * d.re);
@@ -475,16 +511,30 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
     const int rb = (p >= 0.0f) + (q < p);
     const float al = (ra == 0) ? alo : ((ra == 2) ? ahi : amid);
     const float be = (rb == 0) ? alo : ((rb == 2) ? ahi : amid);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      a[j] = __builtin_fmaf(al, fr.x[j], a[j]);
-      a[3 + j] = __builtin_fmaf(be, fr.x[j], a[3 + j]);
-      a[6 + j] = __builtin_fmaf(ga, fr.x[j], a[6 + j]);
-    }
+    add_xi(0, al, fr);
+    add_xi(1, be, fr);
+    add_xi(2, ga, fr);
     a[9] += rh;
   }
+  end_row();
 #pragma unroll
   for (int i = 0; i < K; ++i) acc[i] += a[i];
+}
+
+// The adjoint's upstream gradient in polar form, once per band frequency:
+// (2 |g|, arg(g) / (2 pi) in [-1, 0) revolutions), from FP64 hypot and
+// atan2, each rounded once to FP32.  conj(g) exp(-i x) is then
+// |g| exp(-2 pi i (x / (2 pi) + arg(g) / (2 pi))): one add to the phase
+// instead of a complex product per pair.
+template <typename T>
+__global__ __launch_bounds__(256) void grad_polar_kernel(const T* __restrict__ grad, int64_t n,
+                                                         float2* __restrict__ gpol) {
+  const int64_t m = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (m >= n) return;
+  const double gr = grad[2 * m], gi = grad[2 * m + 1];
+  double ph = atan2(gi, gr) * kInv2Pi;  // [-1/2, 1/2]
+  if (ph >= 0.0) ph -= 1.0;
+  gpol[m] = make_float2((float)(2.0 * hypot(gr, gi)), (float)ph);
 }
 
 // Work decomposition of the adjoint (BwdPlan): workgroup w takes face block
@@ -525,7 +575,7 @@ __device__ __forceinline__ BwdChunk bwd_chunk(int w, int64_t n, int nfb, int nch
 template <bool kAF, typename T>
 __global__ __launch_bounds__(256) void meshft_bwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi,
-    const T* __restrict__ grad, int64_t n, int nfb, int nchunk, int levels, int parts,
+    const float2* __restrict__ gpol, int64_t n, int nfb, int nchunk, int levels, int parts,
     double* __restrict__ part /* [nchunk][kFaceSums][nf] */) {
   constexpr int K = kFaceSums<kAF>;
   constexpr size_t kTileBytes = sizeof(FreqRec) * kBwdWaves * 64;
@@ -566,8 +616,9 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
       FreqRec fr;
       fr.X0 = xm[0]; fr.X1 = xm[1]; fr.X2 = xm[2];
       fr.x[0] = (float)fr.X0; fr.x[1] = (float)fr.X1; fr.x[2] = (float)fr.X2;
-      fr.gr = (float)grad[2 * m];
-      fr.gi = (float)grad[2 * m + 1];
+      const float2 gp = gpol[m];
+      fr.gm = gp.x;
+      fr.gph = gp.y;
       if (kAF) {
         const double q = fr.X0 * fr.X0 + fr.X1 * fr.X1 + fr.X2 * fr.X2;
         fr.inv2 = (q > 0.0) ? (float)(1.0 / q) : 0.0f;
@@ -955,6 +1006,7 @@ struct BwdWs {
   double* S;
   double* spart;
   FaceRec* recs;
+  float2* gpol;  // upstream gradient in polar form, per band frequency
   double* part;
   double* G;
   double* cgA;
@@ -979,6 +1031,7 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n) {
   w.S = c.take<double>(2);
   w.spart = c.take<double>(2 * (size_t)ceil_div(f1, 256));
   w.recs = c.take<FaceRec>(f1);
+  w.gpol = c.take<float2>(n > 0 ? n : 1);
   const BwdPlan p = bwd_plan<kAF>(nf, n);
   w.part = c.take<double>((size_t)p.nchunk * kFaceSums<kAF> * f1);
   w.G = c.take<double>(9 * f1);
@@ -1072,11 +1125,14 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
   if (rc != DMIC_OK) return rc;
   const BwdPlan p = bwd_plan<kAF>(nf, n);
   if (n > 0) {
+    hipLaunchKernelGGL((grad_polar_kernel<T>), dim3(ceil_div(n, 256)), dim3(256), 0, st, grad, n,
+                       w.gpol);
+    DMIC_LAUNCH_CHECK();
     const dim3 grid((unsigned)((int64_t)p.nfb * p.nchunk));
     rc = timer_begin(1, st);
     if (rc != DMIC_OK) return rc;
-    hipLaunchKernelGGL((meshft_bwd_kernel<kAF, T>), grid, dim3(256), 0, st, w.recs, nf, xi, grad,
-                       n, p.nfb, p.nchunk, p.levels, p.parts, w.part);
+    hipLaunchKernelGGL((meshft_bwd_kernel<kAF, T>), grid, dim3(256), 0, st, w.recs, nf, xi,
+                       w.gpol, n, p.nfb, p.nchunk, p.levels, p.parts, w.part);
     DMIC_LAUNCH_CHECK();
     rc = timer_end(1, st);
     if (rc != DMIC_OK) return rc;
