@@ -48,19 +48,26 @@ struct __align__(16) FaceRec {
 };
 static_assert(sizeof(FaceRec) == 64, "FaceRec layout");
 
-// Per-frequency record of the adjoint tiles (48 bytes).  Every pair reads the
-// first 32 bytes (two 16-byte LDS broadcasts); X0 and X1 are read only where
-// a band row starts (the vertex phase's xi_0 and xi_1 terms are constant
+// Per-frequency record of the band tiles (48 bytes).  Every pair of the
+// adjoint reads the first 20 bytes (a 16-byte and a 4-byte LDS broadcast),
+// every pair of the forward the first 12; x0, x1, X0 and X1 are read only
+// where a band row starts (the phases' xi_0 and xi_1 terms are constant
 // along a row).
 struct __align__(16) FreqRec {
-  double X2;      // xi_2 in FP64
-  float x[3];     // xi in FP32
-  float gm, gph;  // upstream gradient g = dL/dRe + i dL/dIm in polar form:
-                  // 2 |g| and arg(g) / (2 pi) in [-1, 0) revolutions
-  float inv2;     // autofluorescence variant: 1 / |xi|^2 (0 at xi = 0); else 0
-  double X0, X1;  // xi_0, xi_1 in FP64
+  double X2;       // xi_2 in FP64
+  float x2;        // xi_2 in FP32
+  float gph;       // adjoint: arg(g) / (2 pi) in [-1, 0) revolutions, of the
+  float gm;        //   upstream gradient g = dL/dRe + i dL/dIm, and 2 |g|
+  float x0, x1;    // xi_0, xi_1 in FP32
+  float inv2;      // autofluorescence variant: 1 / |xi|^2 (0 at xi = 0); else 0
+  double X0, X1;   // xi_0, xi_1 in FP64
 };
 static_assert(sizeof(FreqRec) == 48, "FreqRec layout");
+// The first 16 bytes of a FreqRec, read as one LDS broadcast.
+struct __align__(16) FreqHead {
+  double X2;
+  float x2, gph;
+};
 
 
 // -------------------------------------------------------------------- prep
@@ -278,11 +285,11 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
     const T* xm = xi + 3 * m;
     FreqRec fr;
     fr.X0 = xm[0]; fr.X1 = xm[1]; fr.X2 = xm[2];
-    fr.x[0] = (float)fr.X0; fr.x[1] = (float)fr.X1; fr.x[2] = (float)fr.X2;
+    fr.x0 = (float)fr.X0; fr.x1 = (float)fr.X1; fr.x2 = (float)fr.X2;
     fr.gm = fr.gph = 0.0f;
     fr.inv2 = 0.0f;
     sm.rec[lane] = fr;
-    xx = fr.x[0] * fr.x[0] + fr.x[1] * fr.x[1] + fr.x[2] * fr.x[2];
+    xx = fr.x0 * fr.x0 + fr.x1 * fr.x1 + fr.x2 * fr.x2;
     same_row = lane > 0 && xm[-3] == xm[0] && xm[-2] == xm[1];
   }
   const uint64_t cont = __builtin_amdgcn_ballot_w64(same_row);
@@ -319,14 +326,14 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
         const FreqRec& fr = sm.rec[k];
         if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
           row = fma(r.a[1], fr.X1, r.a[0] * fr.X0);
-          prow = __builtin_fmaf(r.eb[1], fr.x[1], r.eb[0] * fr.x[0]);
-          qrow = __builtin_fmaf(r.ec[1], fr.x[1], r.ec[0] * fr.x[0]);
-          if (kAF) nrow = __builtin_fmaf(r.an[1], fr.x[1], r.an[0] * fr.x[0]);
+          prow = __builtin_fmaf(r.eb[1], fr.x1, r.eb[0] * fr.x0);
+          qrow = __builtin_fmaf(r.ec[1], fr.x1, r.ec[0] * fr.x0);
+          if (kAF) nrow = __builtin_fmaf(r.an[1], fr.x1, r.an[0] * fr.x0);
         }
         // the same operations, in the same order, as a full evaluation
         const double rev_a = fma(r.a[2], fr.X2, row);
-        const float p = __builtin_fmaf(r.eb[2], fr.x[2], prow);
-        const float q = __builtin_fmaf(r.ec[2], fr.x[2], qrow);
+        const float p = __builtin_fmaf(r.eb[2], fr.x2, prow);
+        const float q = __builtin_fmaf(r.ec[2], fr.x2, qrow);
         const Sorted so = sort3(p, q);
         cf F;  // F = e_mid N, face value = -2 m F (r.w carries the -2)
         float m;
@@ -338,7 +345,7 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
         acc[0][k] = __builtin_fmaf(wm, F.re, acc[0][k]);
         acc[1][k] = __builtin_fmaf(wm, F.im, acc[1][k]);
         if (kAF) {
-          const float tm = __builtin_fmaf(r.an[2], fr.x[2], nrow) * m;
+          const float tm = __builtin_fmaf(r.an[2], fr.x2, nrow) * m;
           acc[kV - 2][k] = __builtin_fmaf(tm, F.re, acc[kV - 2][k]);
           acc[kV - 1][k] = __builtin_fmaf(tm, F.im, acc[kV - 1][k]);
         }
@@ -444,41 +451,42 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
     }
   };
   // adds v xi to the sums of factor i
-  auto add_xi = [&](int i, float v, const FreqRec& fr) {
+  auto add_xi = [&](int i, float v, const FreqRec& fr, float x2) {
     const int o = i < 3 ? 3 * i : 10;
     if (kRowSums) {
       rs[i] += v;
     } else {
-      a[o] = __builtin_fmaf(v, fr.x[0], a[o]);
-      a[o + 1] = __builtin_fmaf(v, fr.x[1], a[o + 1]);
+      a[o] = __builtin_fmaf(v, fr.x0, a[o]);
+      a[o + 1] = __builtin_fmaf(v, fr.x1, a[o + 1]);
     }
-    a[o + 2] = __builtin_fmaf(v, fr.x[2], a[o + 2]);
+    a[o + 2] = __builtin_fmaf(v, x2, a[o + 2]);
   };
   double row = 0.0;    // A0 X0 + A1 X1 of the current band row
   float prow = 0.0f;   // b0 x0 + b1 x1
   float qrow = 0.0f;   // c0 x0 + c1 x1
-  for (int k = 0; k < cnt; ++k) {
+  auto pair = [&](int k) __attribute__((always_inline)) {
     const FreqRec& fr = tile[k];
+    const FreqHead fh = *reinterpret_cast<const FreqHead*>(&fr);
     if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
       if (kRowSums && k > 0) end_row();
       row = fma(L.A1, fr.X1, L.A0 * fr.X0);
-      prow = __builtin_fmaf(L.b1, fr.x[1], L.b0 * fr.x[0]);
-      qrow = __builtin_fmaf(L.c1, fr.x[1], L.c0 * fr.x[0]);
-      rx0 = fr.x[0];
-      rx1 = fr.x[1];
+      prow = __builtin_fmaf(L.b1, fr.x1, L.b0 * fr.x0);
+      qrow = __builtin_fmaf(L.c1, fr.x1, L.c0 * fr.x0);
+      rx0 = fr.x0;
+      rx1 = fr.x1;
     }
     // the same operations, in the same order, as a full evaluation
-    const double rev_a = fma(L.A2, fr.X2, row);
-    const float p = __builtin_fmaf(L.b2, fr.x[2], prow);
-    const float q = __builtin_fmaf(L.c2, fr.x[2], qrow);
+    const double rev_a = fma(L.A2, fh.X2, row);
+    const float p = __builtin_fmaf(L.b2, fh.x2, prow);
+    const float q = __builtin_fmaf(L.c2, fh.x2, qrow);
     const Sorted so = sort3(p, q);
     cf d, pP, pQ;
     float rD, u;  // u: phase of the median vertex plus arg(g) / (2 pi), in revolutions
     if (!may_be_wide || !wave_is_wide(so.D)) {
-      u = __builtin_fmaf(so.mid, kInv2PiF, reduce_rev(rev_a) + fr.gph);
+      u = __builtin_fmaf(so.mid, kInv2PiF, reduce_rev(rev_a) + fh.gph);
       face_H_grad<false>(so, d, pP, pQ, rD);
     } else {
-      u = mid_rev<true>(rev_a, so.mid) + fr.gph;
+      u = mid_rev<true>(rev_a, so.mid) + fh.gph;
       face_H_grad<true>(so, d, pP, pQ, rD);
     }
     // Contract with conj(g) first, scaled by rD: z = 2 conj(g) e_mid rD
@@ -496,26 +504,38 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
     if (kAF) {
       // z2 = i z / |xi|^2; zt = w z + s z2 with s = xi . an
       const cf z2 = {-z.im * fr.inv2, z.re * fr.inv2};
-      const float sn = __builtin_fmaf(L.n2, fr.x[2], __builtin_fmaf(L.n1, fr.x[1], L.n0 * fr.x[0]));
+      const float sn = __builtin_fmaf(L.n2, fh.x2, __builtin_fmaf(L.n1, fr.x1, L.n0 * fr.x0));
       zt = {__builtin_fmaf(L.fw, z.re, sn * z2.re), __builtin_fmaf(L.fw, z.im, sn * z2.im)};
       rt = __builtin_fmaf(zt.re, d.re, -zt.im * d.im);
       const float nn = __builtin_fmaf(z2.re, d.re, -z2.im * d.im);
-      add_xi(3, nn, fr);
+      add_xi(3, nn, fr, fh.x2);
     }
     const float rtD = rt * rD;
     const float ga = __builtin_fmaf(zt.re, d.im, zt.im * d.re);
     const float alo = __builtin_fmaf(zt.re, pP.re, __builtin_fmaf(-zt.im, pP.im, rtD));
     const float ahi = -__builtin_fmaf(zt.re, pQ.re, __builtin_fmaf(-zt.im, pQ.im, rtD));
     const float amid = ga - alo - ahi;
-    const int ra = (p < 0.0f) + (q < 0.0f);
-    const int rb = (p >= 0.0f) + (q < p);
+    // roles: vertex a (phase 0) is lo, mid or hi as ra = 0, 1, 2; vertex b likewise
+    const bool pn = p < 0.0f;
+    const int ra = pn + (q < 0.0f);
+    const int rb = !pn + (q < p);
     const float al = (ra == 0) ? alo : ((ra == 2) ? ahi : amid);
     const float be = (rb == 0) ? alo : ((rb == 2) ? ahi : amid);
-    add_xi(0, al, fr);
-    add_xi(1, be, fr);
-    add_xi(2, ga, fr);
+    add_xi(0, al, fr, fh.x2);
+    add_xi(1, be, fr, fh.x2);
+    add_xi(2, ga, fr, fh.x2);
     a[9] += rh;
+  };
+  // two pairs per iteration: their LDS reads share one address (not in the
+  // autofluorescence variant, where it would cost a wave per SIMD)
+  int k = 0;
+  if (!kAF) {
+    for (; k + 1 < cnt; k += 2) {
+      pair(k);
+      pair(k + 1);
+    }
   }
+  for (; k < cnt; ++k) pair(k);
   end_row();
 #pragma unroll
   for (int i = 0; i < K; ++i) acc[i] += a[i];
@@ -615,7 +635,7 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
       const T* xm = xi + 3 * m;
       FreqRec fr;
       fr.X0 = xm[0]; fr.X1 = xm[1]; fr.X2 = xm[2];
-      fr.x[0] = (float)fr.X0; fr.x[1] = (float)fr.X1; fr.x[2] = (float)fr.X2;
+      fr.x0 = (float)fr.X0; fr.x1 = (float)fr.X1; fr.x2 = (float)fr.X2;
       const float2 gp = gpol[m];
       fr.gm = gp.x;
       fr.gph = gp.y;
@@ -626,7 +646,7 @@ __global__ __launch_bounds__(256) void meshft_bwd_kernel(
         fr.inv2 = 0.0f;
       }
       tile[lane] = fr;
-      const float q32 = fr.x[0] * fr.x[0] + fr.x[1] * fr.x[1] + fr.x[2] * fr.x[2];
+      const float q32 = fr.x0 * fr.x0 + fr.x1 * fr.x1 + fr.x2 * fr.x2;
       wide = q32 * e2max > kWidePhase2;
       same_row = lane > 0 && xm[-3] == xm[0] && xm[-2] == xm[1];
     }
