@@ -110,14 +110,29 @@ __device__ __forceinline__ float mid_rev(double rev_a, float mid) {
 // cancels for small y (relative error ~5e-7/y^2 from the sin/cos); below
 // kPhiSeries it comes from its series y (1/6 - y^2/60 + y^4/1680 - y^6/90720)
 // (truncation < 3e-8 at |y| = 0.85, where the closed form is within 1.5e-6).
-__device__ __forceinline__ cf dphi_half(float y, float s, float c, float h, float ry) {
-  const float re = h * (h - c);
+// A wave whose lanes all take the same form evaluates only that one
+// (wave-uniform branches); in about a third of the waves every lane takes the
+// closed form, in most the lanes split.
+__device__ __forceinline__ float dphi_im_series(float y) {
   const float y2 = y * y;
   float si = __builtin_fmaf(y2, -1.1022927690e-5f, 5.9523809524e-4f);
   si = __builtin_fmaf(y2, si, -1.6666666667e-2f);
-  si = __builtin_fmaf(y2, si, 1.6666666667e-1f) * y;
-  const float di = __builtin_fmaf(h, c, __builtin_fmaf(s, s, -0.5f)) * ry;
-  return {re, __builtin_fabsf(y) < kPhiSeries ? si : di};
+  return __builtin_fmaf(y2, si, 1.6666666667e-1f) * y;
+}
+__device__ __forceinline__ cf dphi_half(float y, float s, float c, float h, float ry) {
+  const float re = h * (h - c);
+  const bool small = __builtin_fabsf(y) < kPhiSeries;
+  const uint64_t ns = __builtin_amdgcn_ballot_w64(small);
+  float im;
+  if (ns == 0) {
+    im = __builtin_fmaf(h, c, __builtin_fmaf(s, s, -0.5f)) * ry;
+  } else if (ns == __builtin_amdgcn_ballot_w64(true)) {
+    im = dphi_im_series(y);
+  } else {
+    const float di = __builtin_fmaf(h, c, __builtin_fmaf(s, s, -0.5f)) * ry;
+    im = small ? dphi_im_series(y) : di;
+  }
+  return {re, im};
 }
 // Taylor branch: H(P,Q) = sum_k -(-i)^k h_{k-2}(P,Q)/k!, k = 2..8
 // (kHalf: H/2, the same series with every coefficient halved, exactly).
