@@ -102,8 +102,9 @@ int dmic_meshft_forward_noalloc(const float* verts, int64_t nv, const void* face
 /*
  * Workspace bytes dmic_meshft_backward (and dmic_meshft_backward_f64) needs
  * for these sizes on the current device: the adjoint's frequency chunks are
- * sized from the device's resident workgroups (80 B per face and chunk), so
- * query with the device that will run the call current.
+ * sized from the device's resident workgroups (80 B per face and chunk, plus
+ * 8 B per frequency), so query with the device that will run the call
+ * current.
  */
 size_t dmic_meshft_backward_workspace(int64_t nv, int64_t nf, int64_t n);
 
