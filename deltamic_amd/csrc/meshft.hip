@@ -175,16 +175,29 @@ __device__ __forceinline__ float face_extent2(float b0, float b1, float b2, floa
   return __builtin_fmaxf(nb, __builtin_fmaxf(nc, nd));
 }
 
+// Packed face records are padded to whole blocks of 64 faces.
+__host__ __device__ inline int64_t face_blocks64(int64_t nf) { return (nf + 63) / 64; }
+
 template <typename IdxT, typename T>
 __global__ __launch_bounds__(256) void face_pack_kernel(
     const T* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
     const double* __restrict__ cA, const double* __restrict__ S_in, float regint,
     FaceRec* __restrict__ out, int* flag) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= nf) return;
+  if (t >= face_blocks64(nf) * 64) return;
+  FaceRec r;
+  if (t >= nf) {
+    // padding of the last 64-face block: zero weight, all phases 0 (a finite
+    // Taylor-branch pair that adds nothing)
+    r.a[0] = r.a[1] = r.a[2] = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r.eb[d] = r.ec[d] = r.an[d] = 0.0f;
+    r.w = 0.0f;
+    out[t] = r;
+    return;
+  }
   double P[3][3];
   load_face_verts(V, nv, Fc, t, P, flag);
-  FaceRec r;
   const double S = S_in[0];
   const double regvol = S_in[1];
 #pragma unroll
@@ -303,21 +316,15 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
 #pragma unroll
     for (int k = 0; k < kF; ++k) acc[v][k] = 0.0f;
   const int64_t nblk = (nf + 63) / 64;
-  // face record of block b for this lane (past the last face: the last record
-  // again, given zero weight, so every branch stays finite)
-  auto load_face = [&](int64_t b) {
-    const int64_t t = b * 64 + lane;
-    return faces[t < nf ? t : nf - 1];
-  };
   for (int64_t b = 0; b < nblk; ++b) {
     // the frequency records are re-read from LDS per face block: without this
     // compiler barrier they would be hoisted into ~80 registers for the launch
     asm volatile("" ::: "memory");
-    FaceRec r = load_face(b);
-    if (b * 64 + lane >= nf) {
-      r.w = 0.0f;
-      r.an[0] = r.an[1] = r.an[2] = 0.0f;
-    }
+    // the records are padded to whole blocks with zero-weight faces; a
+    // uniform block address plus the lane's offset needs no vector address
+    // arithmetic
+    const FaceRec* blk = faces + b * 64;
+    const FaceRec r = blk[lane];
     double row = 0.0;
     float prow = 0.0f, qrow = 0.0f, nrow = 0.0f;
 #pragma unroll
@@ -1016,7 +1023,7 @@ static FwdWs carve_fwd(Carver& c, int64_t nf) {
   w.cA = c.take<double>(nf > 0 ? nf : 1);
   w.S = c.take<double>(2);
   w.spart = c.take<double>(2 * (size_t)ceil_div(nf > 0 ? nf : 1, 256));
-  w.recs = c.take<FaceRec>(nf > 0 ? nf : 1);
+  w.recs = c.take<FaceRec>(64 * (size_t)face_blocks64(nf > 0 ? nf : 1));
   return w;
 }
 
@@ -1050,7 +1057,7 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n) {
   w.cA = c.take<double>(f1);
   w.S = c.take<double>(2);
   w.spart = c.take<double>(2 * (size_t)ceil_div(f1, 256));
-  w.recs = c.take<FaceRec>(f1);
+  w.recs = c.take<FaceRec>(64 * (size_t)face_blocks64(f1));
   w.gpol = c.take<float2>(n > 0 ? n : 1);
   const BwdPlan p = bwd_plan<kAF>(nf, n);
   w.part = c.take<double>((size_t)p.nchunk * kFaceSums<kAF> * f1);
@@ -1078,7 +1085,8 @@ static int prep_faces(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const 
   DMIC_LAUNCH_CHECK();
   hipLaunchKernelGGL(face_sum_final_kernel, dim3(1), dim3(1024), 0, st, spart, (int64_t)nb, S);
   DMIC_LAUNCH_CHECK();
-  hipLaunchKernelGGL((face_pack_kernel<IdxT, T>), dim3(ceil_div(nf, 256)), dim3(256), 0, st, V, nv,
+  hipLaunchKernelGGL((face_pack_kernel<IdxT, T>), dim3(ceil_div(64 * face_blocks64(nf), 256)),
+                     dim3(256), 0, st, V, nv,
                      Fc, nf, cA, S, regint, recs, flag);
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
