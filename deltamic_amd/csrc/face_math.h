@@ -224,6 +224,33 @@ __device__ __forceinline__ cf face_H(const Sorted& so, float& m) {
   }
   return N;
 }
+// The forward's form of face_H, with the phases in half revolutions
+// (y' = c y, c = 1/(4 pi)): y' is then the argument of the hardware
+// sin/cos of y/2 as it stands, which saves the two products y c per pair.
+// N and m are homogeneous in the phases: N' = c N and m' = 1/(P'Q'D') =
+// m / c^3, so the returned N' m' is N m / c^2; the caller's face weight
+// carries the factor c^2 (kFwdWScale).  The Taylor branch scales P', Q'
+// back to radians and returns m = -1/(2 c^2).  Only for waves whose phase
+// gaps stay below kWidePhase (no argument reduction).
+constexpr float k4PiF = 12.566370614359172954f;
+constexpr float kFwdWScale = 6.33257799e-3f;  // c^2 = 1/(16 pi^2)
+constexpr float kFwdInvWScale = 157.91367041742973f;  // 16 pi^2
+__device__ __forceinline__ cf face_H_halfrev(const Sorted& so, float& m) {
+  cf N;
+  if (so.D < kTaylorD * kInv4PiF) {
+    N = taylor_H(so.P * k4PiF, so.Q * k4PiF);
+    m = -0.5f * kFwdInvWScale;
+  } else {
+    const float P = __builtin_amdgcn_fmed3f(so.P, -kFltMax, -kTinyY * kInv4PiF);
+    const float Q = __builtin_amdgcn_fmed3f(so.Q, kTinyY * kInv4PiF, kFltMax);
+    const float sP = __builtin_amdgcn_sinf(P), cP = __builtin_amdgcn_cosf(P);
+    const float sQ = __builtin_amdgcn_sinf(Q), cQ = __builtin_amdgcn_cosf(Q);
+    const float u = sP * Q, v = sQ * P;
+    N = {__builtin_fmaf(u, sP, -v * sQ), __builtin_fmaf(u, cP, -v * cQ)};
+    m = rcp(P * Q * so.D);
+  }
+  return N;
+}
 // H and its derivatives for the adjoint, returned halved as (d, pP, pQ, rD) with
 //     H/2 = d rD,  (dH/dP)/2 = (pP + d rD) rD,  (dH/dQ)/2 = -(pQ + d rD) rD,
 // i.e. d = (phi(P) - phi(Q))/2, pP = phi'(P)/2, pQ = phi'(Q)/2, rD = 1/D in

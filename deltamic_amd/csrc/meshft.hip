@@ -1,11 +1,12 @@
 // meshFT forward and adjoint kernels for gfx950, and their C-ABI launchers.
 //
 // Forward (replaces Mesh_NUFT_kernel, deltamic/cpp/fourier3d_cuda_kernel.cu:282-311):
-//   one lane per band frequency; faces are staged through LDS in tiles of
-//   256 packed 64-byte records and read as wave-uniform broadcasts.  Each
-//   lane sums w_t F_t over a tile in FP32 and folds every 64 faces into an
-//   FP64 accumulator.  Small meshes or bands split the face range over
-//   blockIdx.y and a fixed-order reduction adds the FP64 partials.
+//   one lane per face; a one-wave workgroup takes 8 consecutive band
+//   frequencies (records in LDS, read as broadcasts) and walks every face in
+//   blocks of 64 packed 64-byte records read from L2.  Each lane sums w_t F_t
+//   per frequency in FP32; every 64 face blocks the sums are added over the
+//   lanes by a fixed butterfly and folded into FP64.  No partials, no second
+//   kernel, deterministic.
 //
 // Adjoint (replaces Grad_Mesh_NUFT_kernel + Additions_kernel, .cu:315-416):
 //   one lane per face; band frequencies (xi, conj of the upstream gradient)
@@ -181,7 +182,7 @@ __host__ __device__ inline int64_t face_blocks64(int64_t nf) { return (nf + 63) 
 template <typename IdxT, typename T>
 __global__ __launch_bounds__(256) void face_pack_kernel(
     const T* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
-    const double* __restrict__ cA, const double* __restrict__ S_in, float regint,
+    const double* __restrict__ cA, const double* __restrict__ S_in, float regint, float wscale,
     FaceRec* __restrict__ out, int* flag) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= face_blocks64(nf) * 64) return;
@@ -206,11 +207,12 @@ __global__ __launch_bounds__(256) void face_pack_kernel(
     r.eb[d] = (float)(P[1][d] - P[0][d]);  // exact in FP64, rounded once
     r.ec[d] = (float)(P[2][d] - P[0][d]);
   }
-  r.w = (S > 0.0) ? (float)(-4.0 * cA[t] / S) : 0.0f;
+  r.w = (S > 0.0) ? (float)(-4.0 * (double)wscale * cA[t] / S) : 0.0f;
   // autofluorescence: nor2_t = nor_t regint / regvol, nor_t = a x b + b x c + c x a
   // = (b - a) x (c - a) (render.py:208,232)
   // the forward's factor -2 is folded in here too, as in r.w
-  const double sc = (regint != 0.0f && regvol != 0.0) ? -2.0 * (double)regint / regvol : 0.0;
+  const double sc =
+      (regint != 0.0f && regvol != 0.0) ? -2.0 * (double)wscale * regint / regvol : 0.0;
   double u[3], q[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -269,10 +271,25 @@ template <bool kAF>
 constexpr int kFwdFreqs = kAF ? 4 : 8;
 constexpr int kFwdFold = 64;
 
+// Per-frequency record of the forward (48 bytes).  Every pair reads the
+// first 12 bytes (the autofluorescence variant 16); the rest is read where a
+// band row starts.  The edge phases are formed in half revolutions from
+// xi / (4 pi) (face_H_halfrev); the unscaled xi serves the wide-phase
+// variant, which forms them in radians as before.
+struct __align__(16) FwdFreqRec {
+  double X2;       // xi_2 in FP64
+  float h2;        // xi_2 / (4 pi) in FP32 (from FP64)
+  float x2;        // xi_2 in FP32
+  float h0, h1;    // xi_0 / (4 pi), xi_1 / (4 pi)
+  float x0, x1;    // xi_0, xi_1 in FP32
+  double X0, X1;   // xi_0, xi_1 in FP64
+};
+static_assert(sizeof(FwdFreqRec) == 48, "FwdFreqRec layout");
+
 template <bool kAF>
 struct FwdLds {
   static constexpr int kV = kAF ? 4 : 2;  // sums per frequency: F (re, im) [+ (xi . an) F]
-  FreqRec rec[kFwdFreqs<kAF>];
+  FwdFreqRec rec[kFwdFreqs<kAF>];
 };
 
 // kAF: autofluorescence variant (fourier_from_mesh_narrowband_autofluorescence,
@@ -296,11 +313,12 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
   if (lane < cntf) {
     const int64_t m = m0 + lane;
     const T* xm = xi + 3 * m;
-    FreqRec fr;
+    FwdFreqRec fr;
     fr.X0 = xm[0]; fr.X1 = xm[1]; fr.X2 = xm[2];
     fr.x0 = (float)fr.X0; fr.x1 = (float)fr.X1; fr.x2 = (float)fr.X2;
-    fr.gm = fr.gph = 0.0f;
-    fr.inv2 = 0.0f;
+    fr.h0 = (float)(fr.X0 * (0.5 * kInv2Pi));
+    fr.h1 = (float)(fr.X1 * (0.5 * kInv2Pi));
+    fr.h2 = (float)(fr.X2 * (0.5 * kInv2Pi));
     sm.rec[lane] = fr;
     xx = fr.x0 * fr.x0 + fr.x1 * fr.x1 + fr.x2 * fr.x2;
     same_row = lane > 0 && xm[-3] == xm[0] && xm[-2] == xm[1];
@@ -330,24 +348,36 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
 #pragma unroll
     for (int k = 0; k < kF; ++k) {
       if (k < cntf) {  // wave-uniform
-        const FreqRec& fr = sm.rec[k];
+        const FwdFreqRec& fr = sm.rec[k];
         if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
           row = fma(r.a[1], fr.X1, r.a[0] * fr.X0);
-          prow = __builtin_fmaf(r.eb[1], fr.x1, r.eb[0] * fr.x0);
-          qrow = __builtin_fmaf(r.ec[1], fr.x1, r.ec[0] * fr.x0);
+          prow = __builtin_fmaf(r.eb[1], fr.h1, r.eb[0] * fr.h0);
+          qrow = __builtin_fmaf(r.ec[1], fr.h1, r.ec[0] * fr.h0);
           if (kAF) nrow = __builtin_fmaf(r.an[1], fr.x1, r.an[0] * fr.x0);
         }
-        // the same operations, in the same order, as a full evaluation
+        // the same operations, in the same order, as a full evaluation;
+        // edge phases in half revolutions
         const double rev_a = fma(r.a[2], fr.X2, row);
-        const float p = __builtin_fmaf(r.eb[2], fr.x2, prow);
-        const float q = __builtin_fmaf(r.ec[2], fr.x2, qrow);
+        const float p = __builtin_fmaf(r.eb[2], fr.h2, prow);
+        const float q = __builtin_fmaf(r.ec[2], fr.h2, qrow);
         const Sorted so = sort3(p, q);
-        cf F;  // F = e_mid N, face value = -2 m F (r.w carries the -2)
+        cf F;  // F = e_mid N, face value = -2 m F / c^2 (r.w carries -2 c^2)
         float m;
-        if (!may_be_wide || !wave_is_wide(so.D))
-          F = cmul(expmi_rev(mid_rev<false>(rev_a, so.mid)), face_H<false>(so, m));
-        else
-          F = cmul(expmi_rev(mid_rev<true>(rev_a, so.mid)), face_H<true>(so, m));
+        if (!may_be_wide || !wave_is_wide(so.D * k4PiF)) {
+          // median phase: 2 mid' revolutions from vertex a
+          const float u = __builtin_fmaf(so.mid, 2.0f, reduce_rev(rev_a));
+          F = cmul(expmi_rev(u), face_H_halfrev(so, m));
+        } else {
+          // wide variant: edge phases in radians from the unscaled xi (the
+          // products y c of the reduction are then exact fmas), m / c^2
+          const float pr = __builtin_fmaf(r.eb[2], fr.x2,
+                                          __builtin_fmaf(r.eb[1], fr.x1, r.eb[0] * fr.x0));
+          const float qr = __builtin_fmaf(r.ec[2], fr.x2,
+                                          __builtin_fmaf(r.ec[1], fr.x1, r.ec[0] * fr.x0));
+          const Sorted sr = sort3(pr, qr);
+          F = cmul(expmi_rev(mid_rev<true>(rev_a, sr.mid)), face_H<true>(sr, m));
+          m *= kFwdInvWScale;
+        }
         const float wm = r.w * m;
         acc[0][k] = __builtin_fmaf(wm, F.re, acc[0][k]);
         acc[1][k] = __builtin_fmaf(wm, F.im, acc[1][k]);
@@ -381,7 +411,7 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
   const int64_t m = m0 + lane;
   double accr = fin[0], acci = fin[1];
   if (kAF) {
-    const FreqRec& fr = sm.rec[lane];
+    const FwdFreqRec& fr = sm.rec[lane];
     const double q = fr.X0 * fr.X0 + fr.X1 * fr.X1 + fr.X2 * fr.X2;
     if (q > 0.0) {
       accr -= fin[kV - 1] / q;
@@ -1077,8 +1107,8 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n) {
 
 template <typename IdxT, typename T>
 static int prep_faces(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const T* coeff,
-                      float regint, int* flag, double* cA, double* S, double* spart,
-                      FaceRec* recs, hipStream_t st) {
+                      float regint, float wscale, int* flag, double* cA, double* S,
+                      double* spart, FaceRec* recs, hipStream_t st) {
   const unsigned nb = ceil_div(nf, 256);
   hipLaunchKernelGGL((face_area_sum_kernel<IdxT, T>), dim3(nb), dim3(256), 0, st, V, nv, Fc, nf,
                      coeff, cA, spart, flag);
@@ -1087,7 +1117,7 @@ static int prep_faces(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const 
   DMIC_LAUNCH_CHECK();
   hipLaunchKernelGGL((face_pack_kernel<IdxT, T>), dim3(ceil_div(64 * face_blocks64(nf), 256)),
                      dim3(256), 0, st, V, nv,
-                     Fc, nf, cA, S, regint, recs, flag);
+                     Fc, nf, cA, S, regint, wscale, recs, flag);
   DMIC_LAUNCH_CHECK();
   return DMIC_OK;
 }
@@ -1115,8 +1145,8 @@ static int forward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, cons
     DMIC_LAUNCH_CHECK();
     return DMIC_OK;
   }
-  int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S,
-                               w.spart, w.recs, st);
+  int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, kFwdWScale, w.flag,
+                               w.cA, w.S, w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
   const dim3 grid(ceil_div(n, kFwdFreqs<kAF>));
   rc = timer_begin(0, st);
@@ -1148,8 +1178,8 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
     DMIC_LAUNCH_CHECK();
     return DMIC_OK;
   }
-  int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, w.flag, w.cA, w.S,
-                               w.spart, w.recs, st);
+  int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, 1.0f, w.flag, w.cA,
+                               w.S, w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
   const BwdPlan p = bwd_plan<kAF>(nf, n);
   if (n > 0) {
