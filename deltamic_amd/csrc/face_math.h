@@ -103,10 +103,11 @@ __device__ __forceinline__ float mid_rev(double rev_a, float mid) {
   return __builtin_fmaf(mid, kInv2PiF, reduce_rev(rev_a));
 }
 // phi'(y)/2, from s = sin(y/2), c = cos(y/2), h = s/y (= k/2 with
-// k = sin(y/2)/(y/2)) and ry = 1/y:
+// k = sin(y/2)/(y/2)), t = h c (which the caller shares with H) and ry = 1/y:
 //     phi'(y) = [(-sin y + k s) + i(-cos y + k c)] / y.
-// Real part / 2 = h (h - c): no cancellation for any y (2h, c -> 1 as y -> 0).
-// Imaginary part / 2 = (h c + s^2 - 1/2)/y (half the spherical Bessel j1(y))
+// Real part / 2 = h^2 - t = h (h - c): no cancellation for any y (2h, c -> 1
+// as y -> 0).
+// Imaginary part / 2 = (t + s^2 - 1/2)/y (half the spherical Bessel j1(y))
 // cancels for small y (relative error ~5e-7/y^2 from the sin/cos); below
 // kPhiSeries it comes from its series y (1/6 - y^2/60 + y^4/1680 - y^6/90720)
 // (truncation < 3e-8 at |y| = 0.85, where the closed form is within 1.5e-6).
@@ -119,18 +120,19 @@ __device__ __forceinline__ float dphi_im_series(float y) {
   si = __builtin_fmaf(y2, si, -1.6666666667e-2f);
   return __builtin_fmaf(y2, si, 1.6666666667e-1f) * y;
 }
-__device__ __forceinline__ cf dphi_half(float y, float s, float c, float h, float ry) {
-  const float re = h * (h - c);
+__device__ __forceinline__ cf dphi_half(float y, float s, float h, float t, float ry) {
+  const float re = __builtin_fmaf(h, h, -t);
   const bool small = __builtin_fabsf(y) < kPhiSeries;
   const uint64_t ns = __builtin_amdgcn_ballot_w64(small);
   float im;
   if (ns == 0) {
-    im = __builtin_fmaf(h, c, __builtin_fmaf(s, s, -0.5f)) * ry;
+    im = (t + __builtin_fmaf(s, s, -0.5f)) * ry;
   } else if (ns == __builtin_amdgcn_ballot_w64(true)) {
     im = dphi_im_series(y);
   } else {
-    const float di = __builtin_fmaf(h, c, __builtin_fmaf(s, s, -0.5f)) * ry;
-    im = small ? dphi_im_series(y) : di;
+    // both forms, selected by the ballot itself (the compare is not redone)
+    const float di = (t + __builtin_fmaf(s, s, -0.5f)) * ry;
+    im = __builtin_amdgcn_inverse_ballot_w64(ns) ? dphi_im_series(y) : di;
   }
   return {re, im};
 }
@@ -278,9 +280,10 @@ __device__ __forceinline__ void face_H_grad(const Sorted& so, cf& d, cf& pP, cf&
     const float rP = Q * Dr, rQ = P * Dr;   // 1/P, 1/Q
     rD = PQ * r;
     const float hP = sP * rP, hQ = sQ * rQ;
-    d = {hQ * sQ - hP * sP, hQ * cQ - hP * cP};
-    pP = dphi_half(P, sP, cP, hP, rP);
-    pQ = dphi_half(Q, sQ, cQ, hQ, rQ);
+    const float tP = hP * cP, tQ = hQ * cQ;  // shared by d and phi'
+    d = {hQ * sQ - hP * sP, tQ - tP};
+    pP = dphi_half(P, sP, hP, tP, rP);
+    pQ = dphi_half(Q, sQ, hQ, tQ, rQ);
   }
 }
 }  // namespace dmic
