@@ -21,7 +21,7 @@
 //     dF/dx_lo = e_m dH/dP,  dF/dx_hi = e_m dH/dQ,
 //     dF/dx_mid = -iF - dF/dx_lo - dF/dx_hi,
 //     dH/dP = (phi'(P) + H)/D,  dH/dQ = -(phi'(Q) + H)/D,
-// with Im phi'(y) from its series for |y| < 0.85 and in closed form otherwise.
+// with Im phi'(y) from its series for |y| < 0.5 and in closed form otherwise.
 //
 // The phase of vertex a is computed in revolutions in FP64 from xi (FP32,
 // exact in FP64) and a / (2 pi) (FP64, relative error 1e-16), and reduced
@@ -49,7 +49,7 @@ constexpr double kInv2Pi = 0.159154943091895335768883763372514362;
 constexpr float kInv4PiF = 0.0795774715459476678844418816862571810f;
 constexpr float kInv4PiLoF = 3.2103191632826267e-09f;  // 1/(4 pi) - kInv4PiF
 constexpr float kTaylorD = 0.1f;    // largest phase gap handled by the Taylor branch
-constexpr float kPhiSeries = 0.85f;  // |y| below which Im phi'(y) uses its series
+constexpr float kPhiSeries = 0.5f;  // |y| below which Im phi'(y) uses its series
 constexpr float kWidePhase = 16.0f;  // phase gap above which a wave range-reduces its arguments
 // P and Q are kept at least kTinyY away from 0 (phi moves by < 7e-9 there),
 // so one reciprocal of P Q D gives 1/P, 1/Q and 1/D.
@@ -108,16 +108,20 @@ __device__ __forceinline__ float mid_rev(double rev_a, float mid) {
 // Real part / 2 = h^2 - t = h (h - c): no cancellation for any y (2h, c -> 1
 // as y -> 0).
 // Imaginary part / 2 = (t + s^2 - 1/2)/y (half the spherical Bessel j1(y))
-// cancels for small y (relative error ~5e-7/y^2 from the sin/cos); below
-// kPhiSeries it comes from its series y (1/6 - y^2/60 + y^4/1680 - y^6/90720)
-// (truncation < 3e-8 at |y| = 0.85, where the closed form is within 1.5e-6).
+// cancels for small y: the sin/cos errors (1.2e-7) give it an absolute
+// error of ~1.2e-7/y^2, i.e. ~5e-7/y^2 of |phi'/2| (>= 1/4).  Below
+// kPhiSeries = 0.5 it comes from its series y (1/6 - y^2/60 + y^4/1680)
+// (truncation y^7/90720 < 9e-8, 3.5e-7 of |phi'/2|, at |y| = 0.5, where
+// the closed form is within 1.9e-6 of it).  The threshold trades that error for
+// fewer waves whose lanes split between the two forms: on the 20,480-face
+// sphere 53 % of the waves split at 0.5 against 67 % at 0.85, where the
+// series needed a fourth term.
 // A wave whose lanes all take the same form evaluates only that one
-// (wave-uniform branches); in about a third of the waves every lane takes the
-// closed form, in most the lanes split.
+// (wave-uniform branches); at 0.5, 46 % of those waves take the closed form
+// on every lane.
 __device__ __forceinline__ float dphi_im_series(float y) {
   const float y2 = y * y;
-  float si = __builtin_fmaf(y2, -1.1022927690e-5f, 5.9523809524e-4f);
-  si = __builtin_fmaf(y2, si, -1.6666666667e-2f);
+  const float si = __builtin_fmaf(y2, 5.9523809524e-4f, -1.6666666667e-2f);
   return __builtin_fmaf(y2, si, 1.6666666667e-1f) * y;
 }
 __device__ __forceinline__ cf dphi_half(float y, float s, float h, float t, float ry) {
@@ -235,7 +239,9 @@ __device__ __forceinline__ cf face_H(const Sorted& so, float& m) {
 // back to radians and returns m = -1/(2 c^2).  Only for waves whose phase
 // gaps stay below kWidePhase (no argument reduction).
 constexpr float k4PiF = 12.566370614359172954f;
-constexpr float kFwdWScale = 6.33257799e-3f;  // c^2 = 1/(16 pi^2)
+// c^2 = 1/(16 pi^2) in FP64: the packing kernel forms the weight in FP64 and
+// rounds it once, so the forward's values carry no scale error of their own
+constexpr double kFwdWScale = (0.5 * kInv2Pi) * (0.5 * kInv2Pi);
 constexpr float kFwdInvWScale = 157.91367041742973f;  // 16 pi^2
 __device__ __forceinline__ cf face_H_halfrev(const Sorted& so, float& m) {
   cf N;

@@ -182,7 +182,7 @@ __host__ __device__ inline int64_t face_blocks64(int64_t nf) { return (nf + 63) 
 template <typename IdxT, typename T>
 __global__ __launch_bounds__(256) void face_pack_kernel(
     const T* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
-    const double* __restrict__ cA, const double* __restrict__ S_in, float regint, float wscale,
+    const double* __restrict__ cA, const double* __restrict__ S_in, float regint, double wscale,
     FaceRec* __restrict__ out, int* flag) {
   const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (t >= face_blocks64(nf) * 64) return;
@@ -207,12 +207,12 @@ __global__ __launch_bounds__(256) void face_pack_kernel(
     r.eb[d] = (float)(P[1][d] - P[0][d]);  // exact in FP64, rounded once
     r.ec[d] = (float)(P[2][d] - P[0][d]);
   }
-  r.w = (S > 0.0) ? (float)(-4.0 * (double)wscale * cA[t] / S) : 0.0f;
+  r.w = (S > 0.0) ? (float)(-4.0 * wscale * cA[t] / S) : 0.0f;
   // autofluorescence: nor2_t = nor_t regint / regvol, nor_t = a x b + b x c + c x a
   // = (b - a) x (c - a) (render.py:208,232)
   // the forward's factor -2 is folded in here too, as in r.w
   const double sc =
-      (regint != 0.0f && regvol != 0.0) ? -2.0 * (double)wscale * regint / regvol : 0.0;
+      (regint != 0.0f && regvol != 0.0) ? -2.0 * wscale * regint / regvol : 0.0;
   double u[3], q[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -1107,7 +1107,7 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n) {
 
 template <typename IdxT, typename T>
 static int prep_faces(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, const T* coeff,
-                      float regint, float wscale, int* flag, double* cA, double* S,
+                      float regint, double wscale, int* flag, double* cA, double* S,
                       double* spart, FaceRec* recs, hipStream_t st) {
   const unsigned nb = ceil_div(nf, 256);
   hipLaunchKernelGGL((face_area_sum_kernel<IdxT, T>), dim3(nb), dim3(256), 0, st, V, nv, Fc, nf,
@@ -1178,7 +1178,7 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
     DMIC_LAUNCH_CHECK();
     return DMIC_OK;
   }
-  int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, 1.0f, w.flag, w.cA,
+  int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, 1.0, w.flag, w.cA,
                                w.S, w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
   const BwdPlan p = bwd_plan<kAF>(nf, n);
