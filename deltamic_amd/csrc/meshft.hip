@@ -297,17 +297,31 @@ struct FwdLds {
 // lane also sums (xi . an_t) F_t and i/|xi|^2 times it is added at the end;
 // at xi = 0 that term is its limit -regint (the solid's transform at 0 over
 // regvol; the reference divides by zero there).
+// Work split (FwdPlan): the first `head` workgroups take kF band entries each
+// (whole rounds of the device's resident waves); the rest of the band, less
+// than one round's worth, is spread evenly over the `tail` last workgroups
+// (at most kF entries each), so the launch does not end on a partly filled
+// round of full-length waves.
 template <bool kAF, typename T>
 __global__ __launch_bounds__(64) void meshft_fwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi, int64_t n,
-    T* __restrict__ out /* [n][2] */, const double* __restrict__ S_in,
-    const int* __restrict__ e2bits, float regint) {
+    int64_t head, int64_t tail, T* __restrict__ out /* [n][2] */,
+    const double* __restrict__ S_in, const int* __restrict__ e2bits, float regint) {
   constexpr int kV = FwdLds<kAF>::kV;
   constexpr int kF = kFwdFreqs<kAF>;
   __shared__ FwdLds<kAF> sm;
   const int lane = (int)threadIdx.x;
-  const int64_t m0 = (int64_t)blockIdx.x * kF;
-  const int cntf = (int)((n - m0 < kF) ? n - m0 : kF);
+  const int64_t g = blockIdx.x;
+  int64_t m0;
+  int cntf;
+  if (g < head) {
+    m0 = g * kF;
+    cntf = kF;
+  } else {
+    const int64_t base = head * kF, rem = n - base, j = g - head;
+    m0 = base + rem * j / tail;
+    cntf = (int)(base + rem * (j + 1) / tail - m0);
+  }
   bool same_row = false;
   float xx = 0.0f;
   if (lane < cntf) {
@@ -959,11 +973,12 @@ static int timer_end(int k, hipStream_t st) {
 }
 
 
-// Workgroups of `kernel` (256 threads) resident on the whole current device:
-// CUs x workgroups per CU from the runtime's occupancy calculator, cached per
-// device and kernel.  Without a device (workspace queries on a host with no
-// GPU) the MI355X figure, 256 CUs x fallback_per_cu, is used.
-static int resident_wgs(const void* kernel, int fallback_per_cu) {
+// Workgroups of `kernel` (`threads` threads each) resident on the whole
+// current device: CUs x workgroups per CU from the runtime's occupancy
+// calculator, cached per device and kernel.  Without a device (workspace
+// queries on a host with no GPU) the MI355X figure, 256 CUs x
+// fallback_per_cu, is used.
+static int resident_wgs(const void* kernel, int fallback_per_cu, int threads = 256) {
   constexpr int kSlots = 16;
   static std::mutex mu;
   static int dev_of[kSlots], val_of[kSlots];
@@ -980,7 +995,7 @@ static int resident_wgs(const void* kernel, int fallback_per_cu) {
   int cus = 0, per = 0;
   int v = 256 * fallback_per_cu;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) == hipSuccess &&
       cus > 0 && per > 0)
     v = cus * per;
   else
@@ -1131,10 +1146,12 @@ static int forward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, cons
   FwdWs w = carve_fwd(c, nf);
   DMIC_REQUIRE(ws_bytes >= c.bytes(), DMIC_ERR_WORKSPACE,
                "forward workspace too small: %zu < %zu bytes", ws_bytes, c.bytes());
-  // one workgroup per kFwdFreqs band entries on a 1-D grid
-  DMIC_REQUIRE(n <= (int64_t)0x7fffffff * kFwdFreqs<kAF>, DMIC_ERR_ARG,
+  // about one 64-thread workgroup per kFwdFreqs band entries on a 1-D grid,
+  // whose size in work-items must fit in 32 bits
+  constexpr int64_t kMaxFwdGroups = 0xffffffffLL / 64 - 65536;
+  DMIC_REQUIRE(n / kFwdFreqs<kAF> <= kMaxFwdGroups, DMIC_ERR_ARG,
                "band of %lld frequencies exceeds the forward's grid (%lld)", (long long)n,
-               (long long)0x7fffffff * kFwdFreqs<kAF>);
+               (long long)(kMaxFwdGroups * kFwdFreqs<kAF>));
   hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, w.flag, 4, 0);
   DMIC_LAUNCH_CHECK();
   if (n == 0) return DMIC_OK;
@@ -1148,11 +1165,18 @@ static int forward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, cons
   int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, kFwdWScale, w.flag,
                                w.cA, w.S, w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
-  const dim3 grid(ceil_div(n, kFwdFreqs<kAF>));
+  // whole rounds of the resident waves with kF entries per wave, then one
+  // round over which the remainder is spread (kFwdFreqs 8: 7 waves per SIMD)
+  constexpr int64_t kF = kFwdFreqs<kAF>;
+  const int64_t W = resident_wgs((const void*)meshft_fwd_kernel<kAF, T>, kAF ? 32 : 28, 64);
+  const int64_t head = (n / (kF * W)) * W;
+  const int64_t rem = n - head * kF;
+  const int64_t tail = rem > 0 ? (rem < W ? rem : W) : 0;
+  const dim3 grid((unsigned)(head + tail));
   rc = timer_begin(0, st);
   if (rc != DMIC_OK) return rc;
-  hipLaunchKernelGGL((meshft_fwd_kernel<kAF, T>), grid, dim3(64), 0, st, w.recs, nf, xi, n, out,
-                     w.S, w.flag + 1, regint);
+  hipLaunchKernelGGL((meshft_fwd_kernel<kAF, T>), grid, dim3(64), 0, st, w.recs, nf, xi, n, head,
+                     tail, out, w.S, w.flag + 1, regint);
   DMIC_LAUNCH_CHECK();
   rc = timer_end(0, st);
   if (rc != DMIC_OK) return rc;
