@@ -643,8 +643,12 @@ __device__ __forceinline__ BwdChunk bwd_chunk(int w, int64_t n, int nfb, int nch
   return c;
 }
 
+// Compiled for 6 waves per SIMD (80 VGPRs): the few registers this costs are
+// spilled outside the pair loop (once per tile and per launch), and the sixth
+// wave hides more of the pair loop's latency than the spills cost (4 % fewer
+// cycles than at 92 VGPRs and 5 waves).
 template <bool kAF, typename T>
-__global__ __launch_bounds__(256) void meshft_bwd_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) void meshft_bwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi,
     const float2* __restrict__ gpol, int64_t n, int nfb, int nchunk, int levels, int parts,
     double* __restrict__ part /* [nchunk][kFaceSums][nf] */) {
@@ -1032,7 +1036,7 @@ template <bool kAF>
 static BwdPlan bwd_plan(int64_t nf, int64_t n) {
   BwdPlan p{(int)ceil_div(nf > 0 ? nf : 1, kBwdFaces), 1, 0, 1};
   if (nf <= 0 || n <= 0) return p;
-  const int cap = resident_wgs((const void*)meshft_bwd_kernel<kAF, float>, 5);
+  const int cap = resident_wgs((const void*)meshft_bwd_kernel<kAF, float>, 6);
   constexpr int64_t kMinPiece = kBwdWaves * 64;  // one full tile per wave
   // parts per level: enough workgroups per level to fill the device
   const int64_t parts = (cap + 8 * (int64_t)p.nfb - 1) / (8 * (int64_t)p.nfb);
