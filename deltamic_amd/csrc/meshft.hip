@@ -455,6 +455,14 @@ constexpr int kFaceSums = kAF ? 13 : 10;
 constexpr int kBwdWaves = 4;
 constexpr int kBwdFaces = 64;
 
+// The row-start data of a workgroup's 64 faces, kept in LDS: read only where
+// a band row starts, so those values need no registers across the pair loop.
+struct __align__(16) FaceRow {
+  double A0, A1;               // vertex a / (2 pi), first two coordinates
+  float b0, b1, c0, c1;        // edge vectors, first two coordinates
+};
+static_assert(sizeof(FaceRow) == 32, "FaceRow layout");
+
 // This lane's face in the adjoint (zeros for lanes past the last face).
 struct FaceLane {
   double A0, A1, A2;            // vertex a / (2 pi)
@@ -479,6 +487,7 @@ struct FaceLane {
 template <bool kAF>
 __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t cont,
                                          bool may_be_wide, const FaceLane& L,
+                                         const FaceRow& frow,
                                          double (&acc)[kFaceSums<kAF>]) {
   constexpr int K = kFaceSums<kAF>;
   constexpr bool kRowSums = !kAF;
@@ -520,9 +529,13 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
     const FreqHead fh = *reinterpret_cast<const FreqHead*>(&fr);
     if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
       if (kRowSums && k > 0) end_row();
-      row = fma(L.A1, fr.X1, L.A0 * fr.X0);
-      prow = __builtin_fmaf(L.b1, fr.x1, L.b0 * fr.x0);
-      qrow = __builtin_fmaf(L.c1, fr.x1, L.c0 * fr.x0);
+      // the face's row data is read from LDS here, every time: without this
+      // compiler barrier it would be held in registers for the whole tile
+      asm volatile("" ::: "memory");
+      const FaceRow fw = frow;
+      row = fma(fw.A1, fr.X1, fw.A0 * fr.X0);
+      prow = __builtin_fmaf(fw.b1, fr.x1, fw.b0 * fr.x0);
+      qrow = __builtin_fmaf(fw.c1, fr.x1, fw.c0 * fr.x0);
       rx0 = fr.x0;
       rx1 = fr.x1;
     }
@@ -577,16 +590,9 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
     add_xi(2, ga, fr, fh.x2);
     a[9] += rh;
   };
-  // two pairs per iteration: their LDS reads share one address (not in the
-  // autofluorescence variant, where it would cost a wave per SIMD)
-  int k = 0;
-  if (!kAF) {
-    for (; k + 1 < cnt; k += 2) {
-      pair(k);
-      pair(k + 1);
-    }
-  }
-  for (; k < cnt; ++k) pair(k);
+  // one pair per iteration: two per iteration (one LDS address for both)
+  // need registers that would spill at 6 waves per SIMD
+  for (int k = 0; k < cnt; ++k) pair(k);
   end_row();
 #pragma unroll
   for (int i = 0; i < K; ++i) acc[i] += a[i];
@@ -643,10 +649,11 @@ __device__ __forceinline__ BwdChunk bwd_chunk(int w, int64_t n, int nfb, int nch
   return c;
 }
 
-// Compiled for 6 waves per SIMD (80 VGPRs): the few registers this costs are
-// spilled outside the pair loop (once per tile and per launch), and the sixth
-// wave hides more of the pair loop's latency than the spills cost (4 % fewer
-// cycles than at 92 VGPRs and 5 waves).
+// Compiled for 6 waves per SIMD (79 VGPRs, no spills; the autofluorescence
+// variant 80 with 28 B spilled once per wave), which hides more of the pair
+// loop's latency than 5 waves at 92 VGPRs (5 % fewer cycles).  To fit, the
+// faces' row-start data sits in LDS (FaceRow) and the pair loop takes one
+// pair per iteration (with two, spills stored per tile wrote ~1 GB per launch).
 template <bool kAF, typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) void meshft_bwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi,
@@ -654,8 +661,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     double* __restrict__ part /* [nchunk][kFaceSums][nf] */) {
   constexpr int K = kFaceSums<kAF>;
   constexpr size_t kTileBytes = sizeof(FreqRec) * kBwdWaves * 64;
+  constexpr size_t kRowBytes = sizeof(FaceRow) * kBwdFaces;
   constexpr size_t kRedBytes = sizeof(double) * (kBwdWaves - 1) * K * kBwdFaces;
-  __shared__ __align__(16) unsigned char smem[kTileBytes > kRedBytes ? kTileBytes : kRedBytes];
+  constexpr size_t kLoopBytes = kTileBytes + kRowBytes;
+  __shared__ __align__(16) unsigned char smem[kLoopBytes > kRedBytes ? kLoopBytes : kRedBytes];
   // the wave index as a scalar: the compiler then keeps the frequency loop's
   // bounds and the tile address in scalar registers
   const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -675,7 +684,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     L.fw = -0.5f * r.w;  // the face weight 2 c_t A_t / sum c A (exact)
     L.n0 = -0.5f * r.an[0]; L.n1 = -0.5f * r.an[1]; L.n2 = -0.5f * r.an[2];  // exact
   }
-  const float e2max = wave_max64(face_extent2(L.b0, L.b1, L.b2, L.c0, L.c1, L.c2));
+  // (wave-uniform: kept in a scalar register)
+  const float e2max = __builtin_bit_cast(
+      float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
+                 int, wave_max64(face_extent2(L.b0, L.b1, L.b2, L.c0, L.c1, L.c2)))));
+  // the four waves share the face block: wave 0 stores its row data
+  FaceRow* rows = reinterpret_cast<FaceRow*>(smem + kTileBytes);
+  if (wv == 0) rows[lane] = FaceRow{L.A0, L.A1, L.b0, L.b1, L.c0, L.c1};
+  __syncthreads();
   // this wave's share of the chunk
   const int64_t q0 = ck.m0 + (ck.m1 - ck.m0) * wv / kBwdWaves;
   const int64_t q1 = ck.m0 + (ck.m1 - ck.m0) * (wv + 1) / kBwdWaves;
@@ -708,7 +724,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     const uint64_t cont = __builtin_amdgcn_ballot_w64(same_row);
     const bool may_be_wide = __builtin_amdgcn_ballot_w64(wide) != 0;
     wave_lds_sync();
-    bwd_tile<kAF>(tile, cnt, cont, may_be_wide, L, acc);
+    bwd_tile<kAF>(tile, cnt, cont, may_be_wide, L, rows[lane], acc);
     wave_lds_sync();
   }
   // add the waves' sums in a fixed order (the tiles' LDS is reused)
