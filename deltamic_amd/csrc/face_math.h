@@ -84,6 +84,22 @@ __device__ __forceinline__ float reduce_rev(double r) {
   return (float)__builtin_amdgcn_fract(r);
 }
 
+// The same reduction without the FP64 fract and convert.  When the FP64
+// phase is formed as x = kRevMagic + rev (the offset added to the row term,
+// the FMA of the last term rounding x once) and |rev| < 2^19, x lies in
+// [2^20, 2^21), where the FP64 spacing is 2^-32 revolutions and kRevMagic is
+// an integer: the low 32 bits of x are frac(rev) 2^32.  Read as a signed
+// integer they are the phase in [-1/2, 1/2) revolutions times 2^32, which
+// one conversion rounds to FP32 (the caller applies 2^-32 in its next fma).
+// Callers use it where (max |a/2pi|^2)(max |xi|^2) < kRevMagicLimit2, i.e.
+// |rev| < 2^18, and the FP64 path otherwise.
+constexpr double kRevMagic = 1572864.0;             // 1.5 * 2^20
+constexpr float kRevMagicLimit2 = 6.8719476736e10f;  // (2^18)^2
+constexpr float kTwoM32 = 2.3283064365386963e-10f;   // 2^-32
+__device__ __forceinline__ float rev_frac_scaled(double x) {
+  return (float)__double2loint(x);
+}
+
 // exp(-2 pi i u) for u in revolutions (|u| <= 256) with the hardware sin/cos.
 // Only the common phase factor of a face uses it, where an absolute error
 // just rotates that face's contribution.

@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 #include "face_math.h"
@@ -223,10 +224,13 @@ __global__ __launch_bounds__(256) void face_pack_kernel(
   r.an[1] = (float)(sc * (u[2] * q[0] - u[0] * q[2]));
   r.an[2] = (float)(sc * (u[0] * q[1] - u[1] * q[0]));
   out[t] = r;
-  // largest squared face extent of the mesh (flag[1], zeroed with the index
-  // flag; non-negative floats order like their bit patterns)
+  // largest squared face extent of the mesh (flag[1]) and largest |a / 2 pi|^2
+  // (flag[2], rounded up), zeroed with the index flag; non-negative floats
+  // order like their bit patterns
   atomicMax(flag + 1, __float_as_int(face_extent2(r.eb[0], r.eb[1], r.eb[2], r.ec[0], r.ec[1],
                                                   r.ec[2])));
+  atomicMax(flag + 2, __float_as_int(__double2float_ru(r.a[0] * r.a[0] + r.a[1] * r.a[1] +
+                                                       r.a[2] * r.a[2])));
 }
 
 // -------------------------------------------------------------------- forward
@@ -338,7 +342,10 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
     same_row = lane > 0 && xm[-3] == xm[0] && xm[-2] == xm[1];
   }
   const uint64_t cont = __builtin_amdgcn_ballot_w64(same_row);
-  const bool may_be_wide = wave_max64(xx) * __int_as_float(e2bits[0]) > kWidePhase2;
+  const float xxmax = wave_max64(xx);
+  const bool may_be_wide = xxmax * __int_as_float(e2bits[0]) > kWidePhase2;
+  // vertex phases below 2^18 revolutions: reduced without FP64 fract/convert
+  const bool small_rev = xxmax * __int_as_float(e2bits[1]) < kRevMagicLimit2;
   wave_lds_sync();
   // lane v * kF + k keeps the FP64 total of sum v of frequency k
   double tot = 0.0;
@@ -348,6 +355,11 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
 #pragma unroll
     for (int k = 0; k < kF; ++k) acc[v][k] = 0.0f;
   const int64_t nblk = (nf + 63) / 64;
+  // the face loop in two forms, chosen once per workgroup: kMagic reduces
+  // the vertex phase through rev_frac_scaled (the row term carries
+  // kRevMagic), else through the FP64 fract
+  auto face_loop = [&](auto magic) {
+  constexpr bool kMagic = decltype(magic)::value;
   for (int64_t b = 0; b < nblk; ++b) {
     // the frequency records are re-read from LDS per face block: without this
     // compiler barrier they would be hoisted into ~80 registers for the launch
@@ -364,14 +376,14 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
       if (k < cntf) {  // wave-uniform
         const FwdFreqRec& fr = sm.rec[k];
         if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
-          row = fma(r.a[1], fr.X1, r.a[0] * fr.X0);
+          row = fma(r.a[1], fr.X1, fma(r.a[0], fr.X0, kMagic ? kRevMagic : 0.0));
           prow = __builtin_fmaf(r.eb[1], fr.h1, r.eb[0] * fr.h0);
           qrow = __builtin_fmaf(r.ec[1], fr.h1, r.ec[0] * fr.h0);
           if (kAF) nrow = __builtin_fmaf(r.an[1], fr.x1, r.an[0] * fr.x0);
         }
         // the same operations, in the same order, as a full evaluation;
         // edge phases in half revolutions
-        const double rev_a = fma(r.a[2], fr.X2, row);
+        const double rev_a = fma(r.a[2], fr.X2, row);  // + kRevMagic if kMagic
         const float p = __builtin_fmaf(r.eb[2], fr.h2, prow);
         const float q = __builtin_fmaf(r.ec[2], fr.h2, qrow);
         const Sorted so = sort3(p, q);
@@ -379,7 +391,8 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
         float m;
         if (!may_be_wide || !wave_is_wide(so.D * k4PiF)) {
           // median phase: 2 mid' revolutions from vertex a
-          const float u = __builtin_fmaf(so.mid, 2.0f, reduce_rev(rev_a));
+          const float ua = kMagic ? rev_frac_scaled(rev_a) * kTwoM32 : reduce_rev(rev_a);
+          const float u = __builtin_fmaf(so.mid, 2.0f, ua);
           F = cmul(expmi_rev(u), face_H_halfrev(so, m));
         } else {
           // wide variant: edge phases in radians from the unscaled xi (the
@@ -389,7 +402,9 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
           const float qr = __builtin_fmaf(r.ec[2], fr.x2,
                                           __builtin_fmaf(r.ec[1], fr.x1, r.ec[0] * fr.x0));
           const Sorted sr = sort3(pr, qr);
-          F = cmul(expmi_rev(mid_rev<true>(rev_a, sr.mid)), face_H<true>(sr, m));
+          // (the offset removed exactly: both lie in [2^20, 2^21))
+          const double ra = kMagic ? rev_a - kRevMagic : rev_a;
+          F = cmul(expmi_rev(mid_rev<true>(ra, sr.mid)), face_H<true>(sr, m));
           m *= kFwdInvWScale;
         }
         const float wm = r.w * m;
@@ -416,6 +431,17 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
           acc[v][k] = 0.0f;
         }
     }
+  }
+  };
+  // (the autofluorescence variant keeps the FP64 form only: both forms
+  // together would cost it a wave per SIMD)
+  if constexpr (!kAF) {
+    if (small_rev)
+      face_loop(std::true_type{});
+    else
+      face_loop(std::false_type{});
+  } else {
+    face_loop(std::false_type{});
   }
   // lane k gathers the kV totals of frequency k
   double fin[kV];
@@ -484,7 +510,9 @@ struct FaceLane {
 // (then every pair takes the wave-wide check of wave_is_wide).
 // The FP32 sums of a tile (at most 64 frequencies) are folded into the FP64
 // sums at its end.
-template <bool kAF>
+// kMagic: the vertex phase is reduced through rev_frac_scaled (the row term
+// carries kRevMagic; the caller has checked |a/2pi| |xi| < 2^18 for the tile).
+template <bool kAF, bool kMagic>
 __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t cont,
                                          bool may_be_wide, const FaceLane& L,
                                          const FaceRow& frow,
@@ -533,7 +561,7 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
       // compiler barrier it would be held in registers for the whole tile
       asm volatile("" ::: "memory");
       const FaceRow fw = frow;
-      row = fma(fw.A1, fr.X1, fw.A0 * fr.X0);
+      row = fma(fw.A1, fr.X1, fma(fw.A0, fr.X0, kMagic ? kRevMagic : 0.0));
       prow = __builtin_fmaf(fw.b1, fr.x1, fw.b0 * fr.x0);
       qrow = __builtin_fmaf(fw.c1, fr.x1, fw.c0 * fr.x0);
       rx0 = fr.x0;
@@ -547,10 +575,13 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
     cf d, pP, pQ;
     float rD, u;  // u: phase of the median vertex plus arg(g) / (2 pi), in revolutions
     if (!may_be_wide || !wave_is_wide(so.D)) {
-      u = __builtin_fmaf(so.mid, kInv2PiF, reduce_rev(rev_a) + fh.gph);
+      const float ua = kMagic ? __builtin_fmaf(rev_frac_scaled(rev_a), kTwoM32, fh.gph)
+                              : reduce_rev(rev_a) + fh.gph;
+      u = __builtin_fmaf(so.mid, kInv2PiF, ua);
       face_H_grad<false>(so, d, pP, pQ, rD);
     } else {
-      u = mid_rev<true>(rev_a, so.mid) + fh.gph;
+      // (the offset removed exactly: both lie in [2^20, 2^21))
+      u = mid_rev<true>(kMagic ? rev_a - kRevMagic : rev_a, so.mid) + fh.gph;
       face_H_grad<true>(so, d, pP, pQ, rD);
     }
     // Contract with conj(g) first, scaled by rD: z = 2 conj(g) e_mid rD
@@ -673,7 +704,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
   int fb;
   const BwdChunk ck = bwd_chunk((int)blockIdx.x, n, nfb, nchunk, levels, parts, fb);
   const int chunk = ck.chunk;
-  const int64_t t = (int64_t)fb * kBwdFaces + lane;
+  const int t = fb * kBwdFaces + lane;  // nf < 2^31 / 3 (checked at the boundary)
   const bool active = t < nf;
   FaceLane L = {};
   if (active) {
@@ -688,6 +719,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
   const float e2max = __builtin_bit_cast(
       float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
                  int, wave_max64(face_extent2(L.b0, L.b1, L.b2, L.c0, L.c1, L.c2)))));
+  // largest |a / 2 pi|^2 of the block's faces (wave-uniform), for the bound
+  // on the vertex phases that allows the kMagic reduction
+  const float amax2 = __builtin_bit_cast(
+      float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(
+                 int, wave_max64(__double2float_ru(L.A0 * L.A0 + L.A1 * L.A1 + L.A2 * L.A2)))));
   // the four waves share the face block: wave 0 stores its row data
   FaceRow* rows = reinterpret_cast<FaceRow*>(smem + kTileBytes);
   if (wv == 0) rows[lane] = FaceRow{L.A0, L.A1, L.b0, L.b1, L.c0, L.c1};
@@ -700,7 +736,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
   for (int i = 0; i < K; ++i) acc[i] = 0.0;
   for (int64_t base = q0; base < q1; base += 64) {
     const int cnt = (int)((q1 - base < 64) ? q1 - base : 64);
-    bool wide = false, same_row = false;
+    bool wide = false, same_row = false, big = false;
     if (lane < cnt) {
       const int64_t m = base + lane;
       const T* xm = xi + 3 * m;
@@ -719,12 +755,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
       tile[lane] = fr;
       const float q32 = fr.x0 * fr.x0 + fr.x1 * fr.x1 + fr.x2 * fr.x2;
       wide = q32 * e2max > kWidePhase2;
+      big = !(q32 * amax2 < kRevMagicLimit2);
       same_row = lane > 0 && xm[-3] == xm[0] && xm[-2] == xm[1];
     }
     const uint64_t cont = __builtin_amdgcn_ballot_w64(same_row);
     const bool may_be_wide = __builtin_amdgcn_ballot_w64(wide) != 0;
+    const bool small_rev = __builtin_amdgcn_ballot_w64(big) == 0;
     wave_lds_sync();
-    bwd_tile<kAF>(tile, cnt, cont, may_be_wide, L, rows[lane], acc);
+    if (small_rev)
+      bwd_tile<kAF, true>(tile, cnt, cont, may_be_wide, L, rows[lane], acc);
+    else
+      bwd_tile<kAF, false>(tile, cnt, cont, may_be_wide, L, rows[lane], acc);
     wave_lds_sync();
   }
   // add the waves' sums in a fixed order (the tiles' LDS is reused)
@@ -735,8 +776,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
     for (int i = 0; i < K; ++i) red[((wv - 1) * K + i) * kBwdFaces + lane] = acc[i];
   }
   __syncthreads();
-  if (wv == 0 && active) {
-    double* dst = part + (int64_t)chunk * K * nf + t;
+  // the face index formed again from an opaque copy of the lane id, so that
+  // the compiler does not hold t (spilled at 6 waves) through the loop
+  int lane_c;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(lane_c) : "v"(lane));
+  const int tc = fb * kBwdFaces + lane_c;
+  if (wv == 0 && tc < nf) {
+    double* dst = part + (int64_t)chunk * K * nf + tc;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       double s = acc[i];
@@ -1230,6 +1276,10 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
     hipLaunchKernelGGL((grad_polar_kernel<T>), dim3(ceil_div(n, 256)), dim3(256), 0, st, grad, n,
                        w.gpol);
     DMIC_LAUNCH_CHECK();
+    // grid size in work-items fits in 32 bits
+    DMIC_REQUIRE((int64_t)p.nfb * p.nchunk * 256 <= 0xffffffffLL, DMIC_ERR_ARG,
+                 "mesh of %lld faces exceeds the adjoint's grid (%d face blocks x %d chunks)",
+                 (long long)nf, p.nfb, p.nchunk);
     const dim3 grid((unsigned)((int64_t)p.nfb * p.nchunk));
     rc = timer_begin(1, st);
     if (rc != DMIC_OK) return rc;
