@@ -455,6 +455,53 @@ def test_full_size_cfg2_properties(device):
     assert torch.equal(f2.view(torch.float32), f.view(torch.float32)) and torch.equal(a2, a)
 
 
+def test_far_translated_mesh_vertex_phases(device):
+    """Vertex phases beyond 2^18 revolutions take the kernels' FP64 reduction
+    instead of the fixed-point one: config 1's mesh (vertices rounded to
+    multiples of 2^-8) translated by t ~ 1.2e4 on the same grid, so every
+    translated vertex is exact in FP32, has |a/2pi| |xi| up to ~3e5
+    revolutions on its band (frequencies below ~140 keep the fixed-point
+    reduction: both forms run in one launch).  Checked against the FP64
+    oracle on the translated inputs, against the untranslated forward times
+    exp(-i xi.t), and the adjoint against the untranslated adjoint with the
+    upstream gradient rotated by exp(-i xi.t) (the same function of the
+    vertices)."""
+    from deltamic_amd.fourier3d_cpp import backward_fourier3d_cpp, forward_fourier3d_cpp
+    w = syn.config1()
+    bs, axes, otf, band = _band(w, device)
+    V0 = np.round(_shifted(w, bs).astype(np.float64) * 256.0) / 256.0
+    tt = np.array([8192.0, -8192.0, 4096.0]) + np.array([37.0, -91.0, 13.0]) / 256.0
+    V1 = V0 + tt
+    assert np.array_equal(V1.astype(np.float32).astype(np.float64), V1)
+    X = band.xi_list
+    xn = X.cpu().numpy().astype(np.float64)
+    rev = np.abs(V1).max(axis=0) @ np.abs(xn).T / (2 * np.pi)
+    assert rev.max() > 2.0 ** 18 > rev.min()
+    Fc = torch.tensor(w.faces, device=device)
+    C = torch.tensor(w.coeff, device=device)
+    Vt0 = torch.tensor(V0, dtype=torch.float32, device=device)
+    Vt1 = torch.tensor(V1, dtype=torch.float32, device=device)
+    f0 = forward_fourier3d_cpp(Vt0, Fc, C, X)
+    f1 = forward_fourier3d_cpp(Vt1, Fc, C, X)
+    got1 = f1.cpu().numpy().astype(np.complex128)
+    phase = np.exp(-1j * (xn @ tt))
+    trans = O.rel_l2(got1, phase * f0.cpu().numpy().astype(np.complex128))
+    ref1 = O.acc_forward(V1.astype(np.float32), w.faces, w.coeff, xn.astype(np.float32),
+                         threads=threads())
+    err1 = O.rel_l2(got1, ref1)
+    g = _grad(band.idx.numel(), w.seed + 4)
+    g0 = torch.tensor(g, device=device)
+    g1 = torch.tensor((g * phase).astype(np.complex64), device=device)
+    a0 = backward_fourier3d_cpp(Vt0, Fc, C, g0, f0, X).cpu().numpy().astype(np.float64)
+    a1 = backward_fourier3d_cpp(Vt1, Fc, C, g1, f1, X).cpu().numpy().astype(np.float64)
+    adj = O.rel_l2(a1, a0)
+    print(f"far-translated mesh (max vertex phase {rev.max():.3g} revolutions): forward vs "
+          f"oracle {err1:.2e}, vs phase x untranslated {trans:.2e}; adjoint vs untranslated "
+          f"{adj:.2e}")
+    assert err1 <= FWD_TOL and trans <= FWD_TOL
+    assert adj <= BWD_TOL
+
+
 def test_autofluorescence_forward_config1(device):
     """Autofluorescence variant (render.py:228-318) on config 1's full band
     (raw vertices, unit weights, regint 0.5) against the oracle, through the
