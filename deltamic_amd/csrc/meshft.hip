@@ -359,79 +359,79 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
   // the vertex phase through rev_frac_scaled (the row term carries
   // kRevMagic), else through the FP64 fract
   auto face_loop = [&](auto magic) {
-  constexpr bool kMagic = decltype(magic)::value;
-  for (int64_t b = 0; b < nblk; ++b) {
-    // the frequency records are re-read from LDS per face block: without this
-    // compiler barrier they would be hoisted into ~80 registers for the launch
-    asm volatile("" ::: "memory");
-    // the records are padded to whole blocks with zero-weight faces; a
-    // uniform block address plus the lane's offset needs no vector address
-    // arithmetic
-    const FaceRec* blk = faces + b * 64;
-    const FaceRec r = blk[lane];
-    double row = 0.0;
-    float prow = 0.0f, qrow = 0.0f, nrow = 0.0f;
+    constexpr bool kMagic = decltype(magic)::value;
+    for (int64_t b = 0; b < nblk; ++b) {
+      // the frequency records are re-read from LDS per face block: without this
+      // compiler barrier they would be hoisted into ~80 registers for the launch
+      asm volatile("" ::: "memory");
+      // the records are padded to whole blocks with zero-weight faces; a
+      // uniform block address plus the lane's offset needs no vector address
+      // arithmetic
+      const FaceRec* blk = faces + b * 64;
+      const FaceRec r = blk[lane];
+      double row = 0.0;
+      float prow = 0.0f, qrow = 0.0f, nrow = 0.0f;
 #pragma unroll
-    for (int k = 0; k < kF; ++k) {
-      if (k < cntf) {  // wave-uniform
-        const FwdFreqRec& fr = sm.rec[k];
-        if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
-          row = fma(r.a[1], fr.X1, fma(r.a[0], fr.X0, kMagic ? kRevMagic : 0.0));
-          prow = __builtin_fmaf(r.eb[1], fr.h1, r.eb[0] * fr.h0);
-          qrow = __builtin_fmaf(r.ec[1], fr.h1, r.ec[0] * fr.h0);
-          if (kAF) nrow = __builtin_fmaf(r.an[1], fr.x1, r.an[0] * fr.x0);
-        }
-        // the same operations, in the same order, as a full evaluation;
-        // edge phases in half revolutions
-        const double rev_a = fma(r.a[2], fr.X2, row);  // + kRevMagic if kMagic
-        const float p = __builtin_fmaf(r.eb[2], fr.h2, prow);
-        const float q = __builtin_fmaf(r.ec[2], fr.h2, qrow);
-        const Sorted so = sort3(p, q);
-        cf F;  // F = e_mid N, face value = -2 m F / c^2 (r.w carries -2 c^2)
-        float m;
-        if (!may_be_wide || !wave_is_wide(so.D * k4PiF)) {
-          // median phase: 2 mid' revolutions from vertex a
-          const float ua = kMagic ? rev_frac_scaled(rev_a) * kTwoM32 : reduce_rev(rev_a);
-          const float u = __builtin_fmaf(so.mid, 2.0f, ua);
-          F = cmul(expmi_rev(u), face_H_halfrev(so, m));
-        } else {
-          // wide variant: edge phases in radians from the unscaled xi (the
-          // products y c of the reduction are then exact fmas), m / c^2
-          const float pr = __builtin_fmaf(r.eb[2], fr.x2,
-                                          __builtin_fmaf(r.eb[1], fr.x1, r.eb[0] * fr.x0));
-          const float qr = __builtin_fmaf(r.ec[2], fr.x2,
-                                          __builtin_fmaf(r.ec[1], fr.x1, r.ec[0] * fr.x0));
-          const Sorted sr = sort3(pr, qr);
-          // (the offset removed exactly: both lie in [2^20, 2^21))
-          const double ra = kMagic ? rev_a - kRevMagic : rev_a;
-          F = cmul(expmi_rev(mid_rev<true>(ra, sr.mid)), face_H<true>(sr, m));
-          m *= kFwdInvWScale;
-        }
-        const float wm = r.w * m;
-        acc[0][k] = __builtin_fmaf(wm, F.re, acc[0][k]);
-        acc[1][k] = __builtin_fmaf(wm, F.im, acc[1][k]);
-        if (kAF) {
-          const float tm = __builtin_fmaf(r.an[2], fr.x2, nrow) * m;
-          acc[kV - 2][k] = __builtin_fmaf(tm, F.re, acc[kV - 2][k]);
-          acc[kV - 1][k] = __builtin_fmaf(tm, F.im, acc[kV - 1][k]);
+      for (int k = 0; k < kF; ++k) {
+        if (k < cntf) {  // wave-uniform
+          const FwdFreqRec& fr = sm.rec[k];
+          if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
+            row = fma(r.a[1], fr.X1, fma(r.a[0], fr.X0, kMagic ? kRevMagic : 0.0));
+            prow = __builtin_fmaf(r.eb[1], fr.h1, r.eb[0] * fr.h0);
+            qrow = __builtin_fmaf(r.ec[1], fr.h1, r.ec[0] * fr.h0);
+            if (kAF) nrow = __builtin_fmaf(r.an[1], fr.x1, r.an[0] * fr.x0);
+          }
+          // the same operations, in the same order, as a full evaluation;
+          // edge phases in half revolutions
+          const double rev_a = fma(r.a[2], fr.X2, row);  // + kRevMagic if kMagic
+          const float p = __builtin_fmaf(r.eb[2], fr.h2, prow);
+          const float q = __builtin_fmaf(r.ec[2], fr.h2, qrow);
+          const Sorted so = sort3(p, q);
+          cf F;  // F = e_mid N, face value = -2 m F / c^2 (r.w carries -2 c^2)
+          float m;
+          if (!may_be_wide || !wave_is_wide(so.D * k4PiF)) {
+            // median phase: 2 mid' revolutions from vertex a
+            const float ua = kMagic ? rev_frac_scaled(rev_a) * kTwoM32 : reduce_rev(rev_a);
+            const float u = __builtin_fmaf(so.mid, 2.0f, ua);
+            F = cmul(expmi_rev(u), face_H_halfrev(so, m));
+          } else {
+            // wide variant: edge phases in radians from the unscaled xi (the
+            // products y c of the reduction are then exact fmas), m / c^2
+            const float pr = __builtin_fmaf(r.eb[2], fr.x2,
+                                            __builtin_fmaf(r.eb[1], fr.x1, r.eb[0] * fr.x0));
+            const float qr = __builtin_fmaf(r.ec[2], fr.x2,
+                                            __builtin_fmaf(r.ec[1], fr.x1, r.ec[0] * fr.x0));
+            const Sorted sr = sort3(pr, qr);
+            // (the offset removed exactly: both lie in [2^20, 2^21))
+            const double ra = kMagic ? rev_a - kRevMagic : rev_a;
+            F = cmul(expmi_rev(mid_rev<true>(ra, sr.mid)), face_H<true>(sr, m));
+            m *= kFwdInvWScale;
+          }
+          const float wm = r.w * m;
+          acc[0][k] = __builtin_fmaf(wm, F.re, acc[0][k]);
+          acc[1][k] = __builtin_fmaf(wm, F.im, acc[1][k]);
+          if (kAF) {
+            const float tm = __builtin_fmaf(r.an[2], fr.x2, nrow) * m;
+            acc[kV - 2][k] = __builtin_fmaf(tm, F.re, acc[kV - 2][k]);
+            acc[kV - 1][k] = __builtin_fmaf(tm, F.im, acc[kV - 1][k]);
+          }
         }
       }
+      if ((b + 1) % kFwdFold == 0 || b + 1 == nblk) {
+        // fold: each FP32 sum is added over the 64 lanes (butterfly, fixed
+        // order) and into its owner lane's FP64 total
+#pragma unroll
+        for (int v = 0; v < kV; ++v)
+#pragma unroll
+          for (int k = 0; k < kF; ++k) {
+            float x = acc[v][k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+            if (lane == v * kF + k) tot += (double)x;
+            acc[v][k] = 0.0f;
+          }
+      }
     }
-    if ((b + 1) % kFwdFold == 0 || b + 1 == nblk) {
-      // fold: each FP32 sum is added over the 64 lanes (butterfly, fixed
-      // order) and into its owner lane's FP64 total
-#pragma unroll
-      for (int v = 0; v < kV; ++v)
-#pragma unroll
-        for (int k = 0; k < kF; ++k) {
-          float x = acc[v][k];
-#pragma unroll
-          for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-          if (lane == v * kF + k) tot += (double)x;
-          acc[v][k] = 0.0f;
-        }
-    }
-  }
   };
   // (the autofluorescence variant keeps the FP64 form only: both forms
   // together would cost it a wave per SIMD)
@@ -681,7 +681,7 @@ __device__ __forceinline__ BwdChunk bwd_chunk(int w, int64_t n, int nfb, int nch
 }
 
 // Compiled for 6 waves per SIMD (79 VGPRs, no spills; the autofluorescence
-// variant 80 with 28 B spilled once per wave), which hides more of the pair
+// variant 80 with 24 B spilled once per wave), which hides more of the pair
 // loop's latency than 5 waves at 92 VGPRs (5 % fewer cycles).  To fit, the
 // faces' row-start data sits in LDS (FaceRow) and the pair loop takes one
 // pair per iteration (with two, spills stored per tile wrote ~1 GB per launch).
