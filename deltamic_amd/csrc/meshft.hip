@@ -1258,6 +1258,11 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
   BwdWs w = carve_bwd<kAF>(c, nv, nf, n);
   DMIC_REQUIRE(ws_bytes >= c.bytes(), DMIC_ERR_WORKSPACE,
                "backward workspace too small: %zu < %zu bytes", ws_bytes, c.bytes());
+  const BwdPlan p = bwd_plan<kAF>(nf, n);
+  // the adjoint's grid size in work-items must fit in 32 bits
+  DMIC_REQUIRE((int64_t)p.nfb * p.nchunk * 256 <= 0xffffffffLL, DMIC_ERR_ARG,
+               "mesh of %lld faces exceeds the adjoint's grid (%d face blocks x %d chunks)",
+               (long long)nf, p.nfb, p.nchunk);
   hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, w.flag, 4, 0);
   DMIC_LAUNCH_CHECK();
   if (nv == 0) return DMIC_OK;
@@ -1271,15 +1276,10 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
   int rc = prep_faces<IdxT, T>(V, nv, Fc, nf, coeff, kAF ? regint : 0.0f, 1.0, w.flag, w.cA,
                                w.S, w.spart, w.recs, st);
   if (rc != DMIC_OK) return rc;
-  const BwdPlan p = bwd_plan<kAF>(nf, n);
   if (n > 0) {
     hipLaunchKernelGGL((grad_polar_kernel<T>), dim3(ceil_div(n, 256)), dim3(256), 0, st, grad, n,
                        w.gpol);
     DMIC_LAUNCH_CHECK();
-    // grid size in work-items fits in 32 bits
-    DMIC_REQUIRE((int64_t)p.nfb * p.nchunk * 256 <= 0xffffffffLL, DMIC_ERR_ARG,
-                 "mesh of %lld faces exceeds the adjoint's grid (%d face blocks x %d chunks)",
-                 (long long)nf, p.nfb, p.nchunk);
     const dim3 grid((unsigned)((int64_t)p.nfb * p.nchunk));
     rc = timer_begin(1, st);
     if (rc != DMIC_OK) return rc;

@@ -293,6 +293,11 @@ def test_c_abi_rejects_bad_arguments_before_any_launch():
     rc = L.dmic_gemm_bf16(fake, 0, 1, 1, fake, 0, 1, 1, fake, 1, 16, 16 * 65536, 16, null, null,
                           0, 0, null)
     assert rc == -1 and b"65535 column tiles" in L.dmic_last_error()
+    # the adjoint refuses a mesh whose grid (face blocks x frequency chunks x
+    # 256 threads) would not fit in 32 bits, before any launch: 10^8 faces
+    rc = L.dmic_meshft_backward(fake, 3, fake, 1, 100_000_000, fake, fake, fake, fake, 1_000_000,
+                                fake, fake, 1 << 62, null)
+    assert rc == -1 and b"exceeds the adjoint's grid" in L.dmic_last_error()
     # workspaces: the autofluorescence adjoint keeps three more sums per face
     assert L.dmic_meshft_af_backward_workspace(642, 1280, 22000) > \
         L.dmic_meshft_backward_workspace(642, 1280, 22000)
