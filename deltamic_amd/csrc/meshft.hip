@@ -473,6 +473,12 @@ __global__ __launch_bounds__(64) void meshft_fwd_kernel(
 //   N = sum Re(conj g i F / |xi|^2) xi (derivative w.r.t. an_t).
 template <bool kAF>
 constexpr int kFaceSums = kAF ? 13 : 10;
+// Type of the chunk partials of those sums: each is rounded once to FP32
+// (below the rounding the FP32 tile sums already carry) and the chunks are
+// added in FP64.  The autofluorescence variant keeps FP64 partials: with the
+// conversion its adjoint spills registers inside the frequency loop.
+template <bool kAF>
+using PartT = typename std::conditional<kAF, double, float>::type;
 
 // Adjoint workgroup: kBwdWaves waves on the same 64 faces (one per lane),
 // each wave on its own share of the workgroup's frequency chunk with its
@@ -689,7 +695,7 @@ template <bool kAF, typename T>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) void meshft_bwd_kernel(
     const FaceRec* __restrict__ faces, int64_t nf, const T* __restrict__ xi,
     const float2* __restrict__ gpol, int64_t n, int nfb, int nchunk, int levels, int parts,
-    double* __restrict__ part /* [nchunk][kFaceSums][nf] */) {
+    PartT<kAF>* __restrict__ part /* [nchunk][kFaceSums][nf] */) {
   constexpr int K = kFaceSums<kAF>;
   constexpr size_t kTileBytes = sizeof(FreqRec) * kBwdWaves * 64;
   constexpr size_t kRowBytes = sizeof(FaceRow) * kBwdFaces;
@@ -782,13 +788,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6))) voi
   asm volatile("v_mov_b32 %0, %1" : "=v"(lane_c) : "v"(lane));
   const int tc = fb * kBwdFaces + lane_c;
   if (wv == 0 && tc < nf) {
-    double* dst = part + (int64_t)chunk * K * nf + tc;
+    PartT<kAF>* dst = part + (int64_t)chunk * K * nf + tc;
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       double s = acc[i];
 #pragma unroll
       for (int w = 0; w < kBwdWaves - 1; ++w) s += red[(w * K + i) * kBwdFaces + lane];
-      dst[(int64_t)i * nf] = s;
+      dst[(int64_t)i * nf] = (PartT<kAF>)s;
     }
   }
 }
@@ -808,7 +814,7 @@ template <typename IdxT, bool kAF, typename T>
 __global__ __launch_bounds__(256) void bwd_face_finalize_kernel(
     const T* __restrict__ V, int64_t nv, const IdxT* __restrict__ Fc, int64_t nf,
     const T* __restrict__ coeff, const double* __restrict__ cA, const double* __restrict__ S_in,
-    const double* __restrict__ part, int nchunk, float regint, double* __restrict__ G,
+    const PartT<kAF>* __restrict__ part, int nchunk, float regint, double* __restrict__ G,
     double* __restrict__ cgA, double* __restrict__ wR, double* __restrict__ cgV,
     double* __restrict__ qN, int* flag) {
   constexpr int K = kFaceSums<kAF>;
@@ -818,9 +824,9 @@ __global__ __launch_bounds__(256) void bwd_face_finalize_kernel(
 #pragma unroll
   for (int i = 0; i < K; ++i) s[i] = 0.0;
   for (int c = 0; c < nchunk; ++c) {
-    const double* src = part + (int64_t)c * K * nf + t;
+    const PartT<kAF>* src = part + (int64_t)c * K * nf + t;
 #pragma unroll
-    for (int i = 0; i < K; ++i) s[i] += src[(int64_t)i * nf];
+    for (int i = 0; i < K; ++i) s[i] += (double)src[(int64_t)i * nf];
   }
   double P[3][3];
   load_face_verts(V, nv, Fc, t, P, flag);
@@ -1145,7 +1151,7 @@ struct BwdWs {
   double* spart;
   FaceRec* recs;
   float2* gpol;  // upstream gradient in polar form, per band frequency
-  double* part;
+  void* part;    // chunk partials of the per-face sums, PartT<kAF>
   double* G;
   double* cgA;
   double* wR;
@@ -1171,7 +1177,7 @@ static BwdWs carve_bwd(Carver& c, int64_t nv, int64_t nf, int64_t n) {
   w.recs = c.take<FaceRec>(64 * (size_t)face_blocks64(f1));
   w.gpol = c.take<float2>(n > 0 ? n : 1);
   const BwdPlan p = bwd_plan<kAF>(nf, n);
-  w.part = c.take<double>((size_t)p.nchunk * kFaceSums<kAF> * f1);
+  w.part = c.take<PartT<kAF>>((size_t)p.nchunk * kFaceSums<kAF> * f1);
   w.G = c.take<double>(9 * f1);
   w.cgA = c.take<double>(9 * f1);
   w.wR = c.take<double>(f1);
@@ -1284,18 +1290,21 @@ static int backward_impl(const T* V, int64_t nv, const IdxT* Fc, int64_t nf, con
     rc = timer_begin(1, st);
     if (rc != DMIC_OK) return rc;
     hipLaunchKernelGGL((meshft_bwd_kernel<kAF, T>), grid, dim3(256), 0, st, w.recs, nf, xi,
-                       w.gpol, n, p.nfb, p.nchunk, p.levels, p.parts, w.part);
+                       w.gpol, n, p.nfb, p.nchunk, p.levels, p.parts,
+                       static_cast<PartT<kAF>*>(w.part));
     DMIC_LAUNCH_CHECK();
     rc = timer_end(1, st);
     if (rc != DMIC_OK) return rc;
   } else {
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(2 * K * nf, 256)), dim3(256), 0, st,
-                       reinterpret_cast<int*>(w.part), 2 * K * nf, 0);
+    const int64_t words = K * nf * (int64_t)(sizeof(PartT<kAF>) / 4);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(ceil_div(words, 256)), dim3(256), 0, st,
+                       static_cast<int*>(w.part), words, 0);
     DMIC_LAUNCH_CHECK();
   }
   const int nchunk = (n > 0) ? p.nchunk : 1;
   hipLaunchKernelGGL((bwd_face_finalize_kernel<IdxT, kAF, T>), dim3(ceil_div(nf, 256)), dim3(256), 0,
-                     st, V, nv, Fc, nf, coeff, w.cA, w.S, w.part, nchunk, regint, w.G, w.cgA, w.wR,
+                     st, V, nv, Fc, nf, coeff, w.cA, w.S, static_cast<const PartT<kAF>*>(w.part),
+                     nchunk, regint, w.G, w.cgA, w.wR,
                      w.cgV, w.qN, w.flag);
   DMIC_LAUNCH_CHECK();
   hipLaunchKernelGGL(sum_f64_kernel, dim3(1), dim3(1024), 0, st, w.wR, nf, w.K);
