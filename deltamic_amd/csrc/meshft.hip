@@ -558,21 +558,22 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
   double row = 0.0;    // A0 X0 + A1 X1 of the current band row
   float prow = 0.0f;   // b0 x0 + b1 x1
   float qrow = 0.0f;   // c0 x0 + c1 x1
+  // where a band row starts at record k
+  auto start_row = [&](int k) __attribute__((always_inline)) {
+    const FreqRec& fr = tile[k];
+    // the face's row data is read from LDS here, every time: without this
+    // compiler barrier it would be held in registers for the whole tile
+    asm volatile("" ::: "memory");
+    const FaceRow fw = frow;
+    row = fma(fw.A1, fr.X1, fma(fw.A0, fr.X0, kMagic ? kRevMagic : 0.0));
+    prow = __builtin_fmaf(fw.b1, fr.x1, fw.b0 * fr.x0);
+    qrow = __builtin_fmaf(fw.c1, fr.x1, fw.c0 * fr.x0);
+    rx0 = fr.x0;
+    rx1 = fr.x1;
+  };
   auto pair = [&](int k) __attribute__((always_inline)) {
     const FreqRec& fr = tile[k];
     const FreqHead fh = *reinterpret_cast<const FreqHead*>(&fr);
-    if (!((cont >> k) & 1)) {  // a band row starts (wave-uniform)
-      if (kRowSums && k > 0) end_row();
-      // the face's row data is read from LDS here, every time: without this
-      // compiler barrier it would be held in registers for the whole tile
-      asm volatile("" ::: "memory");
-      const FaceRow fw = frow;
-      row = fma(fw.A1, fr.X1, fma(fw.A0, fr.X0, kMagic ? kRevMagic : 0.0));
-      prow = __builtin_fmaf(fw.b1, fr.x1, fw.b0 * fr.x0);
-      qrow = __builtin_fmaf(fw.c1, fr.x1, fw.c0 * fr.x0);
-      rx0 = fr.x0;
-      rx1 = fr.x1;
-    }
     // the same operations, in the same order, as a full evaluation
     const double rev_a = fma(L.A2, fh.X2, row);
     const float p = __builtin_fmaf(L.b2, fh.x2, prow);
@@ -627,10 +628,19 @@ __device__ __forceinline__ void bwd_tile(const FreqRec* tile, int cnt, uint64_t 
     add_xi(2, ga, fr, fh.x2);
     a[9] += rh;
   };
-  // one pair per iteration: two per iteration (one LDS address for both)
-  // need registers that would spill at 6 waves per SIMD
-  for (int k = 0; k < cnt; ++k) pair(k);
-  end_row();
+  // Row by row: the set bits of ~cont are the row starts (bit 0 always; the
+  // bits from cnt on are clear in cont), so a row's pairs run with no
+  // per-pair test.  One pair per iteration: two per iteration (one LDS
+  // address for both) need registers that would spill at 6 waves per SIMD.
+  const uint64_t starts = ~cont;
+  for (int k = 0; k < cnt;) {
+    start_row(k);
+    const uint64_t next = (k < 63) ? (starts & (~0ull << (k + 1))) : 0ull;
+    int k1 = next ? (int)__builtin_ctzll(next) : cnt;
+    if (k1 > cnt) k1 = cnt;
+    for (; k < k1; ++k) pair(k);
+    end_row();
+  }
 #pragma unroll
   for (int i = 0; i < K; ++i) acc[i] += a[i];
 }
